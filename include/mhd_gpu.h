@@ -1,0 +1,133 @@
+/*
+ * mhd_gpu.h -- C-ABI of the MI355X ideal-MHD step library (libmhd_gpu.so).
+ *
+ * Drop-in for the per-step routines of the reference solver mhd3d3.f90
+ * (tiegcm/mhd3d_mpi_fortran): a Fortran driver binds these entry points with
+ * ISO_C_BINDING (mhd3d_mpi_fortran_amd/fortran/mhd_gpu_mod.f90).  All
+ * arguments are plain pointers and sizes; arrays use the reference layout
+ * (Fortran column-major, x fastest, ghost-inclusive):
+ *
+ *   cell arrays  rho,vx,vy,vz,p,bx,by,bz   (nx,ny,nz)       nx = nx0 + 2*ng
+ *   face arrays  bi (nx+1,ny,nz)  bj (nx,ny+1,nz)  bk (nx,ny,nz+1)
+ *   edge EMFs    ei (nx0,ny0+1,nz0+1) ej (nx0+1,ny0,nz0+1) ek (nx0+1,ny0+1,nz0)
+ *   flux/stress  x-faces (nx0+1,ny0,nz0), y-faces (nx0,ny0+1,nz0), z-faces (nx0,ny0,nz0+1)
+ *   (module fields, mhd3d3.f90:251-317)
+ *
+ * One context owns one block (the whole domain) on one GPU and one HIP
+ * stream.  Host arrays are borrowed for the duration of a call.  Every entry
+ * point returns MHD_OK (0) or a positive error code; the reference stops
+ * with a message instead (mhd3d3.f90:3139-3155, 3596).  Entry points that
+ * touch host arrays are synchronous; the others are ordered on the
+ * context's stream and complete before the next host-visible result.
+ */
+#ifndef MHD_GPU_H
+#define MHD_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mhd_ctx mhd_ctx;
+
+enum {
+    MHD_OK = 0,
+    MHD_ERR_HIP = 1,       /* a HIP runtime call failed */
+    MHD_ERR_PARAM = 2,     /* invalid parameter or size */
+    MHD_ERR_STATE = 3,     /* rho<0, NaN, or p<=0 found by mhd_check_state */
+    MHD_ERR_UNSUPPORTED = 4
+};
+
+enum { MHD_BC_EVEN = 0, MHD_BC_PERIODIC = 2 };
+enum { MHD_FP64 = 0, MHD_MIXED = 1 };
+
+/* scheme parameters: module pars (mhd3d3.f90:33-248) and the namelist deck
+ * groups &grid &comset &free (mhd3d3.f90:152-156) */
+typedef struct {
+    int nx0, ny0, nz0;   /* interior cells of the block (whole domain on one GPU) */
+    int ng;              /* ghost depth = nord/2; PDM needs 4, WENO5/MPWENO5 3, TVD 2 */
+    double gam, cfl, pdmb, eps, alpha, beta;
+    int reconop;         /* 1 PDM, 2 TVD (van Leer), 3 WENO5, 4 MPWENO5 */
+    int stepop;          /* 1 Adams-Bashforth, 2 TVD RK2, 3 TVD RK3 */
+    int bc[3];           /* per axis: MHD_BC_EVEN (reference) or MHD_BC_PERIODIC */
+    int precision;       /* MHD_FP64, or MHD_MIXED: fp32 stencils/fluxes/EMF,
+                            fp64 conserved accumulation and CT update */
+    int surrogate;       /* 0, or 1: learned face reconstruction for the flux
+                            (needs mhd_surrogate_set_weights), PDM fallback */
+    int device;          /* HIP device ordinal */
+} mhd_params;
+
+/* fill p with the reference deck defaults (mhd3d3.in: cfl=.3 pdmb=4 eps=1e-8
+ * alpha=2 beta=4, gam=5/3, nord=8, PDM, RK2, even reflection) */
+void mhd_default_params(mhd_params *p);
+
+/* create/destroy a context; allocates all device state (reference:
+ * alloc_pars + alloc_fields, mhd3d3.f90:222-237, 285-317) */
+int mhd_create(mhd_ctx **ctx, const mhd_params *p);
+void mhd_destroy(mhd_ctx *ctx);
+const char *mhd_error_string(int code);
+
+/* metric dxblock/dyblock/dzblock per index, lengths nx, ny, nz
+ * (init_geom, mhd3d3.f90:3483-3485; tensor-product grid) */
+int mhd_set_geometry(mhd_ctx *ctx, const double *dx, const double *dy, const double *dz);
+
+/* copy the primitive state in / out (full ghost-inclusive arrays).  Any of
+ * the download pointers may be NULL. */
+int mhd_upload_state(mhd_ctx *ctx, const double *rho, const double *vx, const double *vy, const double *vz,
+                     const double *p, const double *bi, const double *bj, const double *bk);
+int mhd_download_state(mhd_ctx *ctx, double *rho, double *vx, double *vy, double *vz, double *p, double *bi,
+                       double *bj, double *bk, double *bx, double *by, double *bz);
+
+/* initialization after init_ic (mhd3d3.f90:3327-3377): ghost fill, get_bxyz
+ * and, for Adams-Bashforth, get_dt then tau=3e-4, tau0=tau, xp=x.
+ * *tau: in = deck tau (read_input sets tau0=tau), out = tau after set-up. */
+int mhd_initialize(mhd_ctx *ctx, double *tau);
+
+/* nsteps calls of `step` (mhd3d3.f90:3585-3600): step_ab | step_rk2 | step_rk3.
+ * RK: *tau in = deck tau (used every step, as the reference does).
+ * AB: tau comes from the CFL reduction each step.  On return *tau is the
+ * tau of the last step and *tau_cfl the CFL dt computed at its start (the
+ * reference warns when tau_cfl < tau0 for RK, mhd3d3.f90:3658-3661). */
+int mhd_step(mhd_ctx *ctx, int nsteps, double *tau, double *tau_cfl);
+
+/* per-routine drop-ins, each acting on the context's current state */
+int mhd_get_dt(mhd_ctx *ctx, double *tau_cfl);   /* get_dt, mhd3d3.f90:1608 */
+int mhd_fill_ghosts(mhd_ctx *ctx);                /* fill_hydro_mpi + fill_mag_mpi, 781-819 */
+int mhd_get_bxyz(mhd_ctx *ctx);                   /* get_bxyz, 2118 */
+int mhd_flux_stress(mhd_ctx *ctx);                /* get_flux_stress, 2385 */
+int mhd_eijk_lfm(mhd_ctx *ctx);                   /* get_eijk_LFM, 1754 */
+
+/* copy a device product to the host:
+ *   which = MHD_FETCH_FLUX + q (q = 8*d + f, d = face direction x/y/z,
+ *           f = rho, rvx, rvy, rvz, eng, stress_x, stress_y, stress_z),
+ *           MHD_FETCH_EI/EJ/EK, MHD_FETCH_VF (cell array |v|+|B|/sqrt(rho)) */
+enum { MHD_FETCH_FLUX = 0, MHD_FETCH_EI = 24, MHD_FETCH_EJ = 25, MHD_FETCH_EK = 26, MHD_FETCH_VF = 27 };
+int mhd_fetch(mhd_ctx *ctx, int which, double *host_out);
+
+/* reconstruct_3d (mhd3d3.f90:833-860) of a host array f(fn1,fn2,fn3) into
+ * fl, fr (lnx,lny,lnz); dir 1-3 truncated, 4-6 untruncated transverse */
+int mhd_reconstruct(mhd_ctx *ctx, int dir, const double *f, int fn1, int fn2, int fn3, int lnx, int lny,
+                    int lnz, double *fl, double *fr);
+
+/* diag_rho_neg / diag_overflow (mhd3d3.f90:3143-3160) plus p<=0:
+ * *flags = 1 rho<0, 2 NaN interior rho, 4 interior p<=0 or NaN;
+ * returns MHD_ERR_STATE when any flag is set */
+int mhd_check_state(mhd_ctx *ctx, int *flags);
+
+int mhd_synchronize(mhd_ctx *ctx);
+
+/* device pointer of a state array (for zero-copy callers):
+ * 0 rho 1 vx 2 vy 3 vz 4 p 5 bx 6 by 7 bz 8 bi 9 bj 10 bk */
+void *mhd_device_pointer(mhd_ctx *ctx, int which);
+
+/* kernel timing with HIP events on the context's stream (no counterpart in
+ * the reference, which only prints dates per frame, mhd3d3.f90:3292-3295).
+ * kernel classes: see MHD_K_*; *ms = summed duration, *launches = count */
+enum { MHD_K_FLUX = 0, MHD_K_EMF = 1, MHD_K_UPDATE = 2, MHD_K_CT = 3, MHD_K_FILL = 4, MHD_K_BXYZ = 5,
+       MHD_K_DT = 6, MHD_K_PREV = 7, MHD_K_SURR = 8, MHD_K_NCLASS = 9 };
+int mhd_profile(mhd_ctx *ctx, int enable);
+int mhd_profile_read(mhd_ctx *ctx, int kclass, int *launches, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

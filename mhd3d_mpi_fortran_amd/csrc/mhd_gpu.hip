@@ -1,0 +1,566 @@
+// C-ABI of the MI355X ideal-MHD step (declarations in include/mhd_gpu.h).
+//
+// One context = one block (the whole domain) on one GPU, one HIP stream, one
+// device arena holding the reference-layout state plus the per-stage
+// products (face fluxes, edge EMFs).  The step sequences below follow
+// step_ab / step_rk2 / step_rk3 (mhd3d3.f90:3602-3792) kernel for kernel.
+#include "../../include/mhd_gpu.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "mhd_launch.hpp"
+#include "mhd_step_kernels.hpp"
+
+using namespace mhd;
+
+namespace {
+
+size_t align_up(size_t n) { return (n + 255) & ~size_t(255); }
+
+}  // namespace
+
+void mhd_default_params(mhd_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->nx0 = p->ny0 = p->nz0 = 16;
+    p->ng = 4;
+    p->gam = 5. / 3.;
+    p->cfl = .3;
+    p->pdmb = 4.;
+    p->eps = 1.e-8;
+    p->alpha = 2.;
+    p->beta = 4.;
+    p->reconop = 1;
+    p->stepop = 2;
+    p->bc[0] = p->bc[1] = p->bc[2] = MHD_BC_EVEN;
+    p->precision = MHD_FP64;
+    p->surrogate = 0;
+    p->device = 0;
+}
+
+const char* mhd_error_string(int code) {
+    switch (code) {
+    case MHD_OK: return "ok";
+    case MHD_ERR_HIP: return "HIP runtime error";
+    case MHD_ERR_PARAM: return "invalid parameter or size";
+    case MHD_ERR_STATE: return "state check failed (rho<0, NaN or p<=0)";
+    case MHD_ERR_UNSUPPORTED: return "unsupported option";
+    default: return "unknown error";
+    }
+}
+
+static int min_ghosts(int reconop) { return reconop == 1 ? 4 : (reconop == 2 ? 2 : 3); }
+
+int mhd_create(mhd_ctx** out, const mhd_params* p) {
+    if (!out || !p) return MHD_ERR_PARAM;
+    *out = nullptr;
+    if (p->reconop < 1 || p->reconop > 4 || p->stepop < 1 || p->stepop > 3) return MHD_ERR_PARAM;
+    if (p->ng < min_ghosts(p->reconop)) return MHD_ERR_PARAM;
+    // the ghost fill copies from owned cells only: each axis needs nx0 >= ng
+    if (p->nx0 < p->ng || p->ny0 < p->ng || p->nz0 < p->ng) return MHD_ERR_PARAM;
+    for (int a = 0; a < 3; ++a)
+        if (p->bc[a] != MHD_BC_EVEN && p->bc[a] != MHD_BC_PERIODIC) return MHD_ERR_PARAM;
+    if (p->precision != MHD_FP64 && p->precision != MHD_MIXED) return MHD_ERR_PARAM;
+    if (p->surrogate && (p->reconop != 1 || p->precision != MHD_MIXED)) return MHD_ERR_UNSUPPORTED;
+    if (hipSetDevice(p->device) != hipSuccess) return MHD_ERR_HIP;
+
+    mhd_ctx* c = new mhd_ctx();
+    c->p = *p;
+    Dims& d = c->d;
+    d.nx0 = p->nx0;
+    d.ny0 = p->ny0;
+    d.nz0 = p->nz0;
+    d.ng = p->ng;
+    d.nx = p->nx0 + 2 * p->ng;
+    d.ny = p->ny0 + 2 * p->ng;
+    d.nz = p->nz0 + 2 * p->ng;
+    c->ncell = (long)d.nx * d.ny * d.nz;
+    c->nbi = (long)(d.nx + 1) * d.ny * d.nz;
+    c->nbj = (long)d.nx * (d.ny + 1) * d.nz;
+    c->nbk = (long)d.nx * d.ny * (d.nz + 1);
+    c->nf[0] = (long)(d.nx0 + 1) * d.ny0 * d.nz0;
+    c->nf[1] = (long)d.nx0 * (d.ny0 + 1) * d.nz0;
+    c->nf[2] = (long)d.nx0 * d.ny0 * (d.nz0 + 1);
+    c->ne[0] = (long)d.nx0 * (d.ny0 + 1) * (d.nz0 + 1);
+    c->ne[1] = (long)(d.nx0 + 1) * d.ny0 * (d.nz0 + 1);
+    c->ne[2] = (long)(d.nx0 + 1) * (d.ny0 + 1) * d.nz0;
+    c->ninterior = (long)d.nx0 * d.ny0 * d.nz0;
+    c->ndtpart = (c->ncell + kBlock - 1) / kBlock;
+    const size_t tf = p->precision == MHD_MIXED ? sizeof(float) : sizeof(double);
+
+    // arena layout
+    std::vector<std::pair<void**, size_t>> plan;
+    auto cell = [&](double** q) { plan.push_back({(void**)q, align_up(c->ncell * sizeof(double))}); };
+    double** cells[] = {&c->rho, &c->vx, &c->vy, &c->vz, &c->pr, &c->bx, &c->by, &c->bz, &c->vf};
+    for (auto q : cells) cell(q);
+    plan.push_back({(void**)&c->bi, align_up(c->nbi * sizeof(double))});
+    plan.push_back({(void**)&c->bj, align_up(c->nbj * sizeof(double))});
+    plan.push_back({(void**)&c->bk, align_up(c->nbk * sizeof(double))});
+    plan.push_back({(void**)&c->dx, align_up(d.nx * sizeof(double))});
+    plan.push_back({(void**)&c->dy, align_up(d.ny * sizeof(double))});
+    plan.push_back({(void**)&c->dz, align_up(d.nz * sizeof(double))});
+    for (int a = 0; a < 3; ++a) {
+        plan.push_back({&c->flux[a], align_up(8 * c->nf[a] * tf)});
+        plan.push_back({&c->emf[a], align_up(c->ne[a] * tf)});
+    }
+    plan.push_back({(void**)&c->dtpart, align_up(c->ndtpart * sizeof(double))});
+    plan.push_back({(void**)&c->scal, align_up(8 * sizeof(double))});
+    plan.push_back({(void**)&c->flags, align_up(64)});
+    if (p->stepop == 1) {
+        for (int a = 0; a < 11; ++a) {
+            const long n = a < 8 ? c->ncell : (a == 8 ? c->nbi : (a == 9 ? c->nbj : c->nbk));
+            plan.push_back({(void**)&c->xp[a], align_up(n * sizeof(double))});
+            plan.push_back({(void**)&c->xh[a], align_up(n * sizeof(double))});
+        }
+        plan.push_back({(void**)&c->vfh, align_up(c->ncell * sizeof(double))});
+    } else {
+        double** prev[] = {&c->rhop, &c->mxp, &c->myp, &c->mzp, &c->engp};
+        for (auto q : prev) cell(q);
+        plan.push_back({(void**)&c->bip, align_up(c->nbi * sizeof(double))});
+        plan.push_back({(void**)&c->bjp, align_up(c->nbj * sizeof(double))});
+        plan.push_back({(void**)&c->bkp, align_up(c->nbk * sizeof(double))});
+    }
+    size_t total = 0;
+    for (auto& e : plan) total += e.second;
+    if (hipMalloc(&c->arena, total) != hipSuccess) {
+        delete c;
+        return MHD_ERR_HIP;
+    }
+    c->arena_bytes = total;
+    size_t off = 0;
+    for (auto& e : plan) {
+        *e.first = c->arena + off;
+        off += e.second;
+    }
+    if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) {
+        hipFree(c->arena);
+        delete c;
+        return MHD_ERR_HIP;
+    }
+    // zero everything once so ghost shells and unused slots hold defined values
+    if (hipMemsetAsync(c->arena, 0, total, c->s) != hipSuccess || hipStreamSynchronize(c->s) != hipSuccess) {
+        mhd_destroy(c);
+        return MHD_ERR_HIP;
+    }
+    *out = c;
+    return MHD_OK;
+}
+
+void mhd_destroy(mhd_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->p.device);
+    if (c->s) hipStreamSynchronize(c->s);
+    for (auto& e : c->evpool) hipEventDestroy(e);
+    for (auto& r : c->evs) {
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    surrogate_free(c);
+    if (c->arena) hipFree(c->arena);
+    if (c->s) hipStreamDestroy(c->s);
+    delete c;
+}
+
+int mhd_set_geometry(mhd_ctx* c, const double* dx, const double* dy, const double* dz) {
+    if (!c || !dx || !dy || !dz) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    if (hipMemcpyAsync(c->dx, dx, c->d.nx * sizeof(double), hipMemcpyHostToDevice, c->s) != hipSuccess ||
+        hipMemcpyAsync(c->dy, dy, c->d.ny * sizeof(double), hipMemcpyHostToDevice, c->s) != hipSuccess ||
+        hipMemcpyAsync(c->dz, dz, c->d.nz * sizeof(double), hipMemcpyHostToDevice, c->s) != hipSuccess)
+        return MHD_ERR_HIP;
+    c->have_geometry = true;
+    return hipStreamSynchronize(c->s) == hipSuccess ? MHD_OK : MHD_ERR_HIP;
+}
+
+int mhd_upload_state(mhd_ctx* c, const double* rho, const double* vx, const double* vy, const double* vz,
+                     const double* p, const double* bi, const double* bj, const double* bk) {
+    if (!c || !rho || !vx || !vy || !vz || !p || !bi || !bj || !bk) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    const size_t cb = c->ncell * sizeof(double);
+    const void* src[8] = {rho, vx, vy, vz, p, bi, bj, bk};
+    void* dst[8] = {c->rho, c->vx, c->vy, c->vz, c->pr, c->bi, c->bj, c->bk};
+    const size_t nb[8] = {cb, cb, cb, cb, cb, c->nbi * sizeof(double), c->nbj * sizeof(double),
+                          c->nbk * sizeof(double)};
+    for (int a = 0; a < 8; ++a)
+        if (hipMemcpyAsync(dst[a], src[a], nb[a], hipMemcpyHostToDevice, c->s) != hipSuccess) return MHD_ERR_HIP;
+    return hipStreamSynchronize(c->s) == hipSuccess ? MHD_OK : MHD_ERR_HIP;
+}
+
+int mhd_download_state(mhd_ctx* c, double* rho, double* vx, double* vy, double* vz, double* p, double* bi,
+                       double* bj, double* bk, double* bx, double* by, double* bz) {
+    if (!c) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    const size_t cb = c->ncell * sizeof(double);
+    void* dst[11] = {rho, vx, vy, vz, p, bi, bj, bk, bx, by, bz};
+    const void* src[11] = {c->rho, c->vx, c->vy, c->vz, c->pr, c->bi, c->bj, c->bk, c->bx, c->by, c->bz};
+    const size_t nb[11] = {cb, cb, cb, cb, cb, c->nbi * sizeof(double), c->nbj * sizeof(double),
+                           c->nbk * sizeof(double), cb, cb, cb};
+    for (int a = 0; a < 11; ++a)
+        if (dst[a] && hipMemcpyAsync(dst[a], src[a], nb[a], hipMemcpyDeviceToHost, c->s) != hipSuccess)
+            return MHD_ERR_HIP;
+    return hipStreamSynchronize(c->s) == hipSuccess ? MHD_OK : MHD_ERR_HIP;
+}
+
+// ---------------------------------------------------------------------------
+// building blocks
+// ---------------------------------------------------------------------------
+static Prim<double> cur_state(mhd_ctx* c) {
+    return Prim<double>{c->rho, c->vx, c->vy, c->vz, c->pr, c->bx, c->by, c->bz};
+}
+static Prim<double> h_state(mhd_ctx* c) {
+    return Prim<double>{c->xh[0], c->xh[1], c->xh[2], c->xh[3], c->xh[4], c->xh[5], c->xh[6], c->xh[7]};
+}
+
+static void do_fill(mhd_ctx* c) {
+    FillSet fs;
+    const Dims& d = c->d;
+    double* arr[8] = {c->rho, c->pr, c->vx, c->vy, c->vz, c->bi, c->bj, c->bk};
+    const int face[8] = {0, 0, 0, 0, 0, 1, 2, 3};
+    long most = 0;
+    const int g2 = 2 * d.ng;
+    for (int a = 0; a < 8; ++a) {
+        FillArr& A = fs.arr[a];
+        A.a = arr[a];
+        A.n[0] = d.nx + (face[a] == 1);
+        A.n[1] = d.ny + (face[a] == 2);
+        A.n[2] = d.nz + (face[a] == 3);
+        A.face = face[a];
+        const long o0 = d.nx0 + (face[a] == 1), o1 = d.ny0 + (face[a] == 2);
+        const long cnt = (long)g2 * A.n[1] * A.n[2] + o0 * g2 * A.n[2] + o0 * o1 * g2;
+        most = cnt > most ? cnt : most;
+    }
+    fs.narr = 8;
+    fs.n0[0] = d.nx0;
+    fs.n0[1] = d.ny0;
+    fs.n0[2] = d.nz0;
+    for (int a = 0; a < 3; ++a) fs.bc[a] = c->p.bc[a];
+    dim3 grid((unsigned)((most + kBlock - 1) / kBlock), 8);
+    launch(c, MHD_K_FILL, grid, dim3(kBlock), k_fill, fs, d.ng);
+}
+
+static void do_bxyz(mhd_ctx* c) {
+    launch(c, MHD_K_BXYZ, blocks(c->ncell), dim3(kBlock), k_bxyz, c->d, (const double*)c->bi,
+           (const double*)c->bj, (const double*)c->bk, (const double*)c->rho, (const double*)c->vx,
+           (const double*)c->vy, (const double*)c->vz, c->bx, c->by, c->bz, c->vf, c->ncell);
+}
+
+// get_dt on the current state; the minimum lands in scal[SC_DTCFL] and, when
+// set_tau, also in scal[SC_TAU] (Adams-Bashforth takes tau from the CFL)
+static void do_dt(mhd_ctx* c, bool set_tau) {
+    launch(c, MHD_K_DT, blocks(c->ncell), dim3(kBlock), k_dt_partial, c->d, c->p.gam, c->p.cfl, cur_state(c),
+           (const double*)c->dx, (const double*)c->dy, (const double*)c->dz, c->dtpart, c->ncell);
+    launch(c, MHD_K_DT, dim3(1), dim3(1024), k_dt_final, (const double*)c->dtpart, c->ndtpart,
+           c->scal + SC_DTCFL, set_tau ? c->scal + SC_TAU : (double*)nullptr);
+}
+
+static void do_prev_rk(mhd_ctx* c) {
+    long n = c->ncell;
+    n = c->nbi > n ? c->nbi : n;
+    n = c->nbj > n ? c->nbj : n;
+    n = c->nbk > n ? c->nbk : n;
+    launch(c, MHD_K_PREV, blocks(n), dim3(kBlock), k_prev_rk, c->p.gam - 1., cur_state(c), c->rhop, c->mxp,
+           c->myp, c->mzp, c->engp, (const double*)c->bi, (const double*)c->bj, (const double*)c->bk, c->bip,
+           c->bjp, c->bkp, c->ncell, c->nbi, c->nbj, c->nbk);
+}
+
+template <typename TF, int NST, int ST>
+static void do_update_t(mhd_ctx* c) {
+    Cells s{c->rho, c->vx, c->vy, c->vz, c->pr};
+    PrevRK pv{c->rhop, c->mxp, c->myp, c->mzp, c->engp};
+    FluxSet<TF> F{(const TF*)c->flux[0], (const TF*)c->flux[1], (const TF*)c->flux[2], c->nf[0], c->nf[1], c->nf[2]};
+    launch(c, MHD_K_UPDATE, blocks(c->ninterior), dim3(kBlock), k_update<NST, ST, TF>, c->d, c->p.gam - 1.,
+           (const double*)(c->scal + SC_TAU), s, pv, F, (const double*)c->dx, (const double*)c->dy,
+           (const double*)c->dz, c->ninterior);
+    Faces b{c->bi, c->bj, c->bk};
+    CFaces bp{c->bip, c->bjp, c->bkp};
+    const long nct = (long)(c->d.nx0 + 1) * c->d.ny0 * c->d.nz0 + (long)c->d.nx0 * (c->d.ny0 + 1) * c->d.nz0 +
+                     (long)c->d.nx0 * c->d.ny0 * (c->d.nz0 + 1);
+    launch(c, MHD_K_CT, blocks(nct), dim3(kBlock), k_ct<NST, ST, TF>, c->d, (const double*)(c->scal + SC_TAU), b,
+           bp, (const TF*)c->emf[0], (const TF*)c->emf[1], (const TF*)c->emf[2], (const double*)c->dx,
+           (const double*)c->dy, (const double*)c->dz);
+}
+
+template <int NST, int ST>
+static void do_update(mhd_ctx* c) {
+    if (c->p.precision == MHD_MIXED) do_update_t<float, NST, ST>(c);
+    else do_update_t<double, NST, ST>(c);
+}
+
+// flux + EMF of a given state (RK: current, AB: extrapolated)
+static void do_ops(mhd_ctx* c, Prim<double> q, const double* bi, const double* bj, const double* bk,
+                   const double* vf) {
+    if (c->p.precision == MHD_MIXED) {
+        launch_emf_fp32(c, q, bi, bj, bk, vf);
+        if (c->p.surrogate) launch_flux_surrogate(c, q);
+        else launch_flux_fp32(c, q);
+    } else {
+        launch_emf_fp64(c, q, bi, bj, bk, vf);
+        launch_flux_fp64(c, q);
+    }
+}
+
+static int sync_status(mhd_ctx* c) {
+    if (c->launch_error != hipSuccess) {
+        c->launch_error = hipSuccess;
+        return MHD_ERR_HIP;
+    }
+    return hipStreamSynchronize(c->s) == hipSuccess ? MHD_OK : MHD_ERR_HIP;
+}
+
+// ---------------------------------------------------------------------------
+// public per-routine entry points
+// ---------------------------------------------------------------------------
+int mhd_fill_ghosts(mhd_ctx* c) {
+    if (!c) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    do_fill(c);
+    return sync_status(c);
+}
+
+int mhd_get_bxyz(mhd_ctx* c) {
+    if (!c) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    do_bxyz(c);
+    return sync_status(c);
+}
+
+int mhd_get_dt(mhd_ctx* c, double* tau_cfl) {
+    if (!c || !c->have_geometry) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    do_dt(c, false);
+    double h = 0.;
+    if (hipMemcpyAsync(&h, c->scal + SC_DTCFL, sizeof(double), hipMemcpyDeviceToHost, c->s) != hipSuccess)
+        return MHD_ERR_HIP;
+    int rc = sync_status(c);
+    if (tau_cfl) *tau_cfl = h;
+    return rc;
+}
+
+int mhd_flux_stress(mhd_ctx* c) {
+    if (!c) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    if (c->p.precision == MHD_MIXED) {
+        if (c->p.surrogate) launch_flux_surrogate(c, cur_state(c));
+        else launch_flux_fp32(c, cur_state(c));
+    } else {
+        launch_flux_fp64(c, cur_state(c));
+    }
+    return sync_status(c);
+}
+
+int mhd_eijk_lfm(mhd_ctx* c) {
+    if (!c) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    launch(c, MHD_K_BXYZ, blocks(c->ncell), dim3(kBlock), k_vf, cur_state(c), c->vf, c->ncell);
+    if (c->p.precision == MHD_MIXED) launch_emf_fp32(c, cur_state(c), c->bi, c->bj, c->bk, c->vf);
+    else launch_emf_fp64(c, cur_state(c), c->bi, c->bj, c->bk, c->vf);
+    return sync_status(c);
+}
+
+int mhd_initialize(mhd_ctx* c, double* tau) {
+    if (!c || !c->have_geometry) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    const double t = tau ? *tau : 0.;
+    c->tau0_host = t;
+    launch(c, MHD_K_PREV, dim3(1), dim3(64), k_set2, c->scal + SC_TAU, t, t);
+    do_fill(c);
+    do_bxyz(c);
+    double out = t;
+    if (c->p.stepop == 1) {
+        do_dt(c, true);
+        // hard-coded Adams-Bashforth start (mhd3d3.f90:3358-3373)
+        out = 3.e-4;
+        launch(c, MHD_K_PREV, dim3(1), dim3(64), k_set2, c->scal + SC_TAU, out, out);
+        double* x[11] = {c->rho, c->vx, c->vy, c->vz, c->pr, c->bx, c->by, c->bz, c->bi, c->bj, c->bk};
+        for (int a = 0; a < 11; ++a) {
+            const long n = a < 8 ? c->ncell : (a == 8 ? c->nbi : (a == 9 ? c->nbj : c->nbk));
+            if (hipMemcpyAsync(c->xp[a], x[a], n * sizeof(double), hipMemcpyDeviceToDevice, c->s) != hipSuccess)
+                return MHD_ERR_HIP;
+        }
+    }
+    if (tau) *tau = out;
+    return sync_status(c);
+}
+
+// step_rk2 / step_rk3 (mhd3d3.f90:3648-3792)
+template <int NST>
+static void step_rk(mhd_ctx* c) {
+    do_dt(c, false);
+    launch(c, MHD_K_PREV, dim3(1), dim3(64), k_set2, c->scal + SC_TAU, c->tau0_host, c->tau0_host);
+    do_prev_rk(c);
+    for (int st = 1; st <= NST; ++st) {
+        do_ops(c, cur_state(c), c->bi, c->bj, c->bk, c->vf);
+        if (st == 1) do_update<NST, 1>(c);
+        else if (st == 2) do_update<NST, 2>(c);
+        else do_update<NST, 3>(c);
+        do_fill(c);
+        do_bxyz(c);
+    }
+}
+
+// step_ab (mhd3d3.f90:3602-3644)
+static void step_ab(mhd_ctx* c) {
+    do_dt(c, true);
+    AB11 s;
+    double* x[11] = {c->rho, c->vx, c->vy, c->vz, c->pr, c->bx, c->by, c->bz, c->bi, c->bj, c->bk};
+    for (int a = 0; a < 11; ++a) {
+        s.x[a] = x[a];
+        s.xp[a] = c->xp[a];
+        s.xh[a] = c->xh[a];
+    }
+    long n = c->ncell;
+    n = c->nbi > n ? c->nbi : n;
+    n = c->nbj > n ? c->nbj : n;
+    n = c->nbk > n ? c->nbk : n;
+    launch(c, MHD_K_PREV, blocks(n), dim3(kBlock), k_predict_ab, s, (const double*)(c->scal + SC_TAU), c->vfh,
+           c->ncell, c->nbi, c->nbj, c->nbk);
+    launch(c, MHD_K_PREV, dim3(1), dim3(64), k_tau0_from_tau, c->scal + SC_TAU);
+    do_ops(c, h_state(c), c->xh[8], c->xh[9], c->xh[10], c->vfh);
+    do_update<0, 0>(c);
+    do_fill(c);
+    do_bxyz(c);
+}
+
+int mhd_step(mhd_ctx* c, int nsteps, double* tau, double* tau_cfl) {
+    if (!c || nsteps < 0 || !c->have_geometry) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    if (c->p.stepop != 1 && tau) c->tau0_host = *tau;
+    for (int n = 0; n < nsteps; ++n) {
+        if (c->p.stepop == 1) step_ab(c);
+        else if (c->p.stepop == 2) step_rk<2>(c);
+        else step_rk<3>(c);
+    }
+    double h[2] = {0., 0.};
+    if (hipMemcpyAsync(h, c->scal, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s) != hipSuccess)
+        return MHD_ERR_HIP;
+    int rc = sync_status(c);
+    if (tau) *tau = h[SC_TAU];
+    if (tau_cfl) *tau_cfl = h[SC_DTCFL];
+    return rc;
+}
+
+int mhd_fetch(mhd_ctx* c, int which, double* out) {
+    if (!c || !out) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    const bool f32 = c->p.precision == MHD_MIXED;
+    const void* src;
+    long n;
+    bool typed = true;
+    if (which >= MHD_FETCH_FLUX && which < MHD_FETCH_FLUX + 24) {
+        const int dq = which / 8, fq = which % 8;
+        n = c->nf[dq];
+        src = (const char*)c->flux[dq] + (size_t)fq * n * (f32 ? 4 : 8);
+    } else if (which >= MHD_FETCH_EI && which <= MHD_FETCH_EK) {
+        n = c->ne[which - MHD_FETCH_EI];
+        src = c->emf[which - MHD_FETCH_EI];
+    } else if (which == MHD_FETCH_VF) {
+        n = c->ncell;
+        src = c->vf;
+        typed = false;
+    } else {
+        return MHD_ERR_PARAM;
+    }
+    if (f32 && typed) {
+        std::vector<float> tmp(n);
+        if (hipMemcpyAsync(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, c->s) != hipSuccess)
+            return MHD_ERR_HIP;
+        int rc = sync_status(c);
+        for (long q = 0; q < n; ++q) out[q] = tmp[q];
+        return rc;
+    }
+    if (hipMemcpyAsync(out, src, n * sizeof(double), hipMemcpyDeviceToHost, c->s) != hipSuccess) return MHD_ERR_HIP;
+    return sync_status(c);
+}
+
+int mhd_reconstruct(mhd_ctx* c, int dir, const double* f, int fn1, int fn2, int fn3, int lnx, int lny, int lnz,
+                    double* fl, double* fr) {
+    if (!c || !f || !fl || !fr || dir < 1 || dir > 6 || fn1 <= 0 || fn2 <= 0 || fn3 <= 0 || lnx <= 0 ||
+        lny <= 0 || lnz <= 0)
+        return MHD_ERR_PARAM;
+    // every stencil tap must stay inside f
+    const int ng = c->p.ng;
+    const int lo = c->p.reconop == 1 ? -4 : (c->p.reconop == 2 ? -2 : -3);
+    const int hi = c->p.reconop == 1 ? 3 : (c->p.reconop == 2 ? 1 : 2);
+    const int ax = dir <= 3 ? dir : dir - 3;
+    const int l[3] = {lnx, lny, lnz}, fn[3] = {fn1, fn2, fn3};
+    for (int a = 0; a < 3; ++a) {
+        const int off = (dir <= 3 || a == ax - 1) ? ng : 0;
+        const int mn = 1 + off + (a == ax - 1 ? lo : 0), mx = l[a] + off + (a == ax - 1 ? hi : 0);
+        if (mn < 1 || mx > fn[a]) return MHD_ERR_PARAM;
+    }
+    hipSetDevice(c->p.device);
+    const size_t nf = (size_t)fn1 * fn2 * fn3, nl = (size_t)lnx * lny * lnz;
+    double *df = nullptr, *dl = nullptr, *dr = nullptr;
+    if (hipMalloc(&df, nf * sizeof(double)) != hipSuccess) return MHD_ERR_HIP;
+    if (hipMalloc(&dl, nl * sizeof(double)) != hipSuccess || hipMalloc(&dr, nl * sizeof(double)) != hipSuccess) {
+        hipFree(df);
+        if (dl) hipFree(dl);
+        return MHD_ERR_HIP;
+    }
+    int rc = MHD_OK;
+    if (hipMemcpyAsync(df, f, nf * sizeof(double), hipMemcpyHostToDevice, c->s) != hipSuccess) rc = MHD_ERR_HIP;
+    if (rc == MHD_OK) {
+        launch_recon(c, dir, df, fn1, fn2, lnx, lny, lnz, dl, dr);
+        if (hipMemcpyAsync(fl, dl, nl * sizeof(double), hipMemcpyDeviceToHost, c->s) != hipSuccess ||
+            hipMemcpyAsync(fr, dr, nl * sizeof(double), hipMemcpyDeviceToHost, c->s) != hipSuccess)
+            rc = MHD_ERR_HIP;
+        int r2 = sync_status(c);
+        if (rc == MHD_OK) rc = r2;
+    }
+    hipFree(df);
+    hipFree(dl);
+    hipFree(dr);
+    return rc;
+}
+
+int mhd_check_state(mhd_ctx* c, int* flags) {
+    if (!c) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    launch(c, MHD_K_DT, dim3(1), dim3(64), k_zero_int<0>, c->flags);
+    launch(c, MHD_K_DT, blocks(c->ncell), dim3(kBlock), k_check, c->d, (const double*)c->rho, (const double*)c->pr,
+           c->flags, c->ncell);
+    int h = 0;
+    if (hipMemcpyAsync(&h, c->flags, sizeof(int), hipMemcpyDeviceToHost, c->s) != hipSuccess) return MHD_ERR_HIP;
+    int rc = sync_status(c);
+    if (flags) *flags = h;
+    if (rc != MHD_OK) return rc;
+    return h ? MHD_ERR_STATE : MHD_OK;
+}
+
+int mhd_synchronize(mhd_ctx* c) {
+    if (!c) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    return sync_status(c);
+}
+
+void* mhd_device_pointer(mhd_ctx* c, int which) {
+    if (!c) return nullptr;
+    void* t[11] = {c->rho, c->vx, c->vy, c->vz, c->pr, c->bx, c->by, c->bz, c->bi, c->bj, c->bk};
+    return which >= 0 && which < 11 ? t[which] : nullptr;
+}
+
+int mhd_profile(mhd_ctx* c, int enable) {
+    if (!c) return MHD_ERR_PARAM;
+    c->prof = enable != 0;
+    return MHD_OK;
+}
+
+int mhd_profile_read(mhd_ctx* c, int kclass, int* launches, double* ms) {
+    if (!c || kclass < 0 || kclass >= MHD_K_NCLASS) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    if (hipStreamSynchronize(c->s) != hipSuccess) return MHD_ERR_HIP;
+    for (auto& r : c->evs) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, r.a, r.b) != hipSuccess) return MHD_ERR_HIP;
+        c->prof_ms[r.kclass] += t;
+        c->prof_n[r.kclass] += 1;
+        c->evpool.push_back(r.a);
+        c->evpool.push_back(r.b);
+    }
+    c->evs.clear();
+    if (launches) *launches = c->prof_n[kclass];
+    if (ms) *ms = c->prof_ms[kclass];
+    return MHD_OK;
+}

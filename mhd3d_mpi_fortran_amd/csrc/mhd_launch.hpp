@@ -1,0 +1,114 @@
+// Context object and launch plumbing shared by the library's translation
+// units (binary64 stencil kernels, fp32 stencil kernels, surrogate).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/mhd_gpu.h"
+#include "mhd_kernels.hpp"
+
+// device scalars: scal[SC_TAU] = tau, scal[SC_TAU0] = tau0, scal[SC_DTCFL] = get_dt result
+enum { SC_TAU = 0, SC_TAU0 = 1, SC_DTCFL = 2 };
+
+struct SurrogateState;
+
+struct mhd_ctx {
+    mhd_params p{};
+    mhd::Dims d{};
+    hipStream_t s = nullptr;
+    char* arena = nullptr;
+    size_t arena_bytes = 0;
+    bool have_geometry = false;
+    long ncell = 0, nbi = 0, nbj = 0, nbk = 0, ninterior = 0, ndtpart = 0;
+    long nf[3] = {0, 0, 0}, ne[3] = {0, 0, 0};
+    // state (reference layout)
+    double *rho = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr, *pr = nullptr;
+    double *bx = nullptr, *by = nullptr, *bz = nullptr, *vf = nullptr;
+    double *bi = nullptr, *bj = nullptr, *bk = nullptr;
+    double *dx = nullptr, *dy = nullptr, *dz = nullptr;
+    // RK advance_previous
+    double *rhop = nullptr, *mxp = nullptr, *myp = nullptr, *mzp = nullptr, *engp = nullptr;
+    double *bip = nullptr, *bjp = nullptr, *bkp = nullptr;
+    // AB previous / extrapolated (rho vx vy vz p bx by bz bi bj bk)
+    double* xp[11] = {};
+    double* xh[11] = {};
+    double* vfh = nullptr;
+    // per-stage products (binary64, or fp32 in mixed precision)
+    void* flux[3] = {nullptr, nullptr, nullptr};
+    void* emf[3] = {nullptr, nullptr, nullptr};
+    double* dtpart = nullptr;
+    double* scal = nullptr;
+    int* flags = nullptr;
+    double tau0_host = 0.;
+    hipError_t launch_error = hipSuccess;
+    // kernel timing
+    bool prof = false;
+    struct Rec {
+        int kclass;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> evs;
+    std::vector<hipEvent_t> evpool;
+    double prof_ms[MHD_K_NCLASS] = {};
+    int prof_n[MHD_K_NCLASS] = {};
+    SurrogateState* sur = nullptr;
+};
+
+inline dim3 blocks(long n) { return dim3((unsigned)((n + mhd::kBlock - 1) / mhd::kBlock)); }
+
+inline hipEvent_t ev_get(mhd_ctx* c) {
+    if (!c->evpool.empty()) {
+        hipEvent_t e = c->evpool.back();
+        c->evpool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+}
+
+template <typename... KArgs, typename... Args>
+inline void launch(mhd_ctx* c, int kclass, dim3 g, dim3 b, void (*k)(KArgs...), Args... args) {
+    if (g.x == 0 || g.y == 0) return;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->prof) {
+        e0 = ev_get(c);
+        e1 = ev_get(c);
+        hipEventRecord(e0, c->s);
+    }
+    hipLaunchKernelGGL(k, g, b, 0, c->s, static_cast<KArgs>(args)...);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess && c->launch_error == hipSuccess) c->launch_error = err;
+    if (c->prof) {
+        hipEventRecord(e1, c->s);
+        c->evs.push_back({kclass, e0, e1});
+    }
+}
+
+template <typename R>
+inline mhd::Coef<R> make_coef(const mhd_ctx* c) {
+    mhd::Coef<R> k;
+    const double pi = 3.141592653589793238462643383279502884;  // binary64 of 4*atan(1)
+    k.gam = R(c->p.gam);
+    k.gm1 = R(c->p.gam - 1.);
+    k.cfl = R(c->p.cfl);
+    k.pdmb = R(c->p.pdmb);
+    k.eps = R(c->p.eps);
+    k.alpha = R(c->p.alpha);
+    k.beta = R(c->p.beta);
+    k.rspi = R(std::sqrt(pi));
+    return k;
+}
+
+// per-precision stencil launchers (separate translation units)
+void launch_flux_fp64(mhd_ctx* c, mhd::Prim<double> q);
+void launch_emf_fp64(mhd_ctx* c, mhd::Prim<double> q, const double* bi, const double* bj, const double* bk,
+                     const double* vf);
+void launch_recon(mhd_ctx* c, int dir, const double* f, int fn1, int fn2, int lnx, int lny, int lnz, double* fl,
+                  double* fr);
+void launch_flux_fp32(mhd_ctx* c, mhd::Prim<double> q);
+void launch_emf_fp32(mhd_ctx* c, mhd::Prim<double> q, const double* bi, const double* bj, const double* bk,
+                     const double* vf);
+void launch_flux_surrogate(mhd_ctx* c, mhd::Prim<double> q);
+void surrogate_free(mhd_ctx* c);

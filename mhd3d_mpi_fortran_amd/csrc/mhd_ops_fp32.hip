@@ -1,0 +1,13 @@
+// fp32 stencil kernels of the mixed-precision step: reconstruction, face
+// flux and edge EMF in fp32 (FMA contraction allowed), stored as fp32; the
+// conserved accumulation and the CT face update stay binary64.
+#include "mhd_ops.hpp"
+
+using namespace mhd;
+
+void launch_flux_fp32(mhd_ctx* c, Prim<double> q) { flux_launch<float, float>(c, q); }
+
+void launch_emf_fp32(mhd_ctx* c, Prim<double> q, const double* bi, const double* bj, const double* bk,
+                     const double* vf) {
+    emf_launch<float, float>(c, q, bi, bj, bk, vf);
+}

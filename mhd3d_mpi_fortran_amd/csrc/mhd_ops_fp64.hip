@@ -1,0 +1,37 @@
+// binary64 stencil kernels (compiled with -ffp-contract=off: same operation
+// order as the reference, no fused multiply-add)
+#include "mhd_ops.hpp"
+
+using namespace mhd;
+
+void launch_flux_fp64(mhd_ctx* c, Prim<double> q) { flux_launch<double, double>(c, q); }
+
+void launch_emf_fp64(mhd_ctx* c, Prim<double> q, const double* bi, const double* bj, const double* bk,
+                     const double* vf) {
+    emf_launch<double, double>(c, q, bi, bj, bk, vf);
+}
+
+void launch_recon(mhd_ctx* c, int dir, const double* f, int fn1, int fn2, int lnx, int lny, int lnz, double* fl,
+                  double* fr) {
+    const Coef<double> co = make_coef<double>(c);
+    const long n = (long)lnx * lny * lnz;
+    const int ng = c->p.ng;
+    switch (c->p.reconop) {
+    case PDM:
+        launch(c, MHD_K_FLUX, blocks(n), dim3(kBlock), k_recon<double, PDM>, ng, dir, f, fn1, fn2, lnx, lny, lnz,
+               co, fl, fr);
+        break;
+    case TVD:
+        launch(c, MHD_K_FLUX, blocks(n), dim3(kBlock), k_recon<double, TVD>, ng, dir, f, fn1, fn2, lnx, lny, lnz,
+               co, fl, fr);
+        break;
+    case WENO5:
+        launch(c, MHD_K_FLUX, blocks(n), dim3(kBlock), k_recon<double, WENO5>, ng, dir, f, fn1, fn2, lnx, lny,
+               lnz, co, fl, fr);
+        break;
+    default:
+        launch(c, MHD_K_FLUX, blocks(n), dim3(kBlock), k_recon<double, MPWENO5>, ng, dir, f, fn1, fn2, lnx, lny,
+               lnz, co, fl, fr);
+        break;
+    }
+}

@@ -1,0 +1,444 @@
+// Cell/face/ghost/reduction kernels of the step: the conserved update and
+// stage combine, constrained transport, ghost fill, get_bxyz, get_dt, the
+// time-stepper bookkeeping and the state diagnostics.  Included by the
+// library's main translation unit only (binary64, no contraction).
+#pragma once
+#include "mhd_kernels.hpp"
+
+namespace mhd {
+
+// ------------------------------------------------------------------------
+// conserved update of one interior cell: get_conserve + advance_corrector1
+// (+ stage combine + pressure) + advance_corrector2 (+ combine + velocity)
+// (mhd3d3.f90:1641-1650, 3842-3907, 3648-3792).
+//   NST = 0: Adams-Bashforth corrector (no combine, p from the new momentum)
+//   NST = 2/3: TVD RK stage ST
+// ------------------------------------------------------------------------
+struct Cells {  // mutable cell state (in place)
+    double *rho, *vx, *vy, *vz, *p;
+};
+struct PrevRK {  // advance_previous, RK flavour
+    const double *rho, *mx, *my, *mz, *eng;
+};
+
+template <typename TF>
+struct FluxSet {
+    const TF *x, *y, *z;
+    long nfx, nfy, nfz;
+};
+
+template <int NST, int ST, typename TF>
+__global__ void __launch_bounds__(kBlock) k_update(Dims d, double gm1, const double* __restrict__ tau_p, Cells s,
+                                                  PrevRK pv, FluxSet<TF> F, const double* __restrict__ dx,
+                                                  const double* __restrict__ dy, const double* __restrict__ dz,
+                                                  long ncell) {
+    const long t = gtid();
+    if (t >= ncell) return;
+    int i, j, k;
+    unflat(t, d.nx0, d.ny0, i, j, k);
+    const int ng = d.ng;
+    const ptrdiff_t q = ix(d.nx, d.ny, ng + i, ng + j, ng + k);
+    const double tau = *tau_p;
+    const double dxi = dx[ng + i - 1], dyj = dy[ng + j - 1], dzk = dz[ng + k - 1];
+    const ptrdiff_t fx0 = ix(d.nx0 + 1, d.ny0, i, j, k);
+    const ptrdiff_t fy0 = ix(d.nx0, d.ny0 + 1, i, j, k), fy1 = fy0 + d.nx0;
+    const ptrdiff_t fz0 = ix(d.nx0, d.ny0, i, j, k), fz1 = fz0 + (ptrdiff_t)d.nx0 * d.ny0;
+    auto div = [&](int f) -> double {
+        return (double(F.x[f * F.nfx + fx0 + 1]) - double(F.x[f * F.nfx + fx0])) / dxi +
+               (double(F.y[f * F.nfy + fy1]) - double(F.y[f * F.nfy + fy0])) / dyj +
+               (double(F.z[f * F.nfz + fz1]) - double(F.z[f * F.nfz + fz0])) / dzk;
+    };
+    // get_conserve from the stage's primitives
+    const double r0 = s.rho[q], u = s.vx[q], v = s.vy[q], w = s.vz[q];
+    const double mx0 = r0 * u, my0 = r0 * v, mz0 = r0 * w;
+    const double e0 = .5 * r0 * (u * u + v * v + w * w) + s.p[q] / gm1;
+    double rho = r0 - tau * div(0);
+    double eng = e0 - tau * div(4);
+    double mx = mx0 - tau * div(1);
+    double my = my0 - tau * div(2);
+    double mz = mz0 - tau * div(3);
+    double hx = mx, hy = my, hz = mz;
+    if (NST == 2 && ST == 2) {
+        rho = (pv.rho[q] + rho) / 2.;
+        hx = (pv.mx[q] + mx) / 2.;
+        hy = (pv.my[q] + my) / 2.;
+        hz = (pv.mz[q] + mz) / 2.;
+        eng = (pv.eng[q] + eng) / 2.;
+    } else if (NST == 3 && ST == 2) {
+        rho = (3. * pv.rho[q] + rho) / 4.;
+        hx = (3. * pv.mx[q] + mx) / 4.;
+        hy = (3. * pv.my[q] + my) / 4.;
+        hz = (3. * pv.mz[q] + mz) / 4.;
+        eng = (3. * pv.eng[q] + eng) / 4.;
+    } else if (NST == 3 && ST == 3) {
+        rho = (pv.rho[q] + 2. * rho) / 3.;
+        hx = (pv.mx[q] + 2. * mx) / 3.;
+        hy = (pv.my[q] + 2. * my) / 3.;
+        hz = (pv.mz[q] + 2. * mz) / 3.;
+        eng = (pv.eng[q] + 2. * eng) / 3.;
+    }
+    // pressure from the hydro-only momentum (3685, 3876)
+    const double p = gm1 * (eng - .5 * (hx * hx + hy * hy + hz * hz) / rho);
+    mx = mx - tau * div(5);
+    my = my - tau * div(6);
+    mz = mz - tau * div(7);
+    if (NST == 2 && ST == 2) {
+        mx = (pv.mx[q] + mx) / 2.;
+        my = (pv.my[q] + my) / 2.;
+        mz = (pv.mz[q] + mz) / 2.;
+    } else if (NST == 3 && ST == 2) {
+        mx = (3. * pv.mx[q] + mx) / 4.;
+        my = (3. * pv.my[q] + my) / 4.;
+        mz = (3. * pv.mz[q] + mz) / 4.;
+    } else if (NST == 3 && ST == 3) {
+        mx = (pv.mx[q] + 2. * mx) / 3.;
+        my = (pv.my[q] + 2. * my) / 3.;
+        mz = (pv.mz[q] + 2. * mz) / 3.;
+    }
+    s.rho[q] = rho;
+    s.p[q] = p;
+    s.vx[q] = mx / rho;
+    s.vy[q] = my / rho;
+    s.vz[q] = mz / rho;
+}
+
+// ------------------------------------------------------------------------
+// constrained transport: advance_bijk (mhd3d3.f90:3909-3933) + face combine
+// one thread per owned face of bi, then bj, then bk
+// ------------------------------------------------------------------------
+struct Faces {
+    double *bi, *bj, *bk;
+};
+struct CFaces {
+    const double *bi, *bj, *bk;
+};
+
+template <int NST, int ST, typename TF>
+__global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict__ tau_p, Faces b, CFaces bp,
+                                              const TF* __restrict__ ei, const TF* __restrict__ ej,
+                                              const TF* __restrict__ ek, const double* __restrict__ dx,
+                                              const double* __restrict__ dy, const double* __restrict__ dz) {
+    const long t = gtid();
+    const int nx0 = d.nx0, ny0 = d.ny0, nz0 = d.nz0, ng = d.ng;
+    const long nbi = (long)(nx0 + 1) * ny0 * nz0, nbj = (long)nx0 * (ny0 + 1) * nz0,
+               nbk = (long)nx0 * ny0 * (nz0 + 1);
+    if (t >= nbi + nbj + nbk) return;
+    const double tau = *tau_p;
+    int i, j, k;
+    double val, old;
+    ptrdiff_t q;
+    double* out;
+    const double* prev;
+    if (t < nbi) {
+        unflat(t, nx0 + 1, ny0, i, j, k);
+        // ek (nx0+1,ny0+1,nz0), ej (nx0+1,ny0,nz0+1)
+        const ptrdiff_t a = ix(nx0 + 1, ny0 + 1, i, j, k), e = ix(nx0 + 1, ny0, i, j, k);
+        const double dek = double(ek[a + (nx0 + 1)]) - double(ek[a]);
+        const double dej = double(ej[e + (ptrdiff_t)(nx0 + 1) * ny0]) - double(ej[e]);
+        q = ix(d.nx + 1, d.ny, ng + i, ng + j, ng + k);
+        old = b.bi[q];
+        val = old - tau * (dek / dy[ng + j - 1] - dej / dz[ng + k - 1]);
+        out = b.bi;
+        prev = bp.bi;
+    } else if (t < nbi + nbj) {
+        unflat(t - nbi, nx0, ny0 + 1, i, j, k);
+        // ei (nx0,ny0+1,nz0+1), ek (nx0+1,ny0+1,nz0)
+        const ptrdiff_t a = ix(nx0, ny0 + 1, i, j, k), e = ix(nx0 + 1, ny0 + 1, i, j, k);
+        const double dei = double(ei[a + (ptrdiff_t)nx0 * (ny0 + 1)]) - double(ei[a]);
+        const double dek = double(ek[e + 1]) - double(ek[e]);
+        q = ix(d.nx, d.ny + 1, ng + i, ng + j, ng + k);
+        old = b.bj[q];
+        val = old - tau * (dei / dz[ng + k - 1] - dek / dx[ng + i - 1]);
+        out = b.bj;
+        prev = bp.bj;
+    } else {
+        unflat(t - nbi - nbj, nx0, ny0, i, j, k);
+        // ej (nx0+1,ny0,nz0+1), ei (nx0,ny0+1,nz0+1)
+        const ptrdiff_t a = ix(nx0 + 1, ny0, i, j, k), e = ix(nx0, ny0 + 1, i, j, k);
+        const double dej = double(ej[a + 1]) - double(ej[a]);
+        const double dei = double(ei[e + nx0]) - double(ei[e]);
+        q = ix(d.nx, d.ny, ng + i, ng + j, ng + k);
+        old = b.bk[q];
+        val = old - tau * (dej / dx[ng + i - 1] - dei / dy[ng + j - 1]);
+        out = b.bk;
+        prev = bp.bk;
+    }
+    (void)old;
+    if (NST == 2 && ST == 2) val = (prev[q] + val) / 2.;
+    else if (NST == 3 && ST == 2) val = (3. * prev[q] + val) / 4.;
+    else if (NST == 3 && ST == 3) val = (prev[q] + 2. * val) / 3.;
+    out[q] = val;
+}
+
+// ------------------------------------------------------------------------
+// ghost fill of the whole block (fill_hydro_mpi + fill_mag_mpi,
+// mhd3d3.f90:346-616, 781-819).  The reference's x, then y, then z fill over
+// full extents composes to one map per axis: a ghost (i,j,k) copies the owned
+// element (map_x(i), map_y(j), map_z(k)).  One thread per ghost element; every
+// source is owned in all three axes, so no thread reads what another writes.
+// ------------------------------------------------------------------------
+struct FillArr {
+    double* a;
+    int n[3];     // extents
+    int face;     // 0 cells, 1..3 normal axis of a face array
+};
+struct FillSet {
+    FillArr arr[8];
+    int narr;
+    int n0[3];
+    int bc[3];    // 0 even reflection, 2 periodic
+};
+
+__device__ __forceinline__ int fill_map(int i, int n0, int ng, bool face, int bc) {
+    const int hi = ng + n0 + (face ? 1 : 0);
+    if (i <= ng) return bc == 2 ? i + n0 : (face ? 2 * ng + 2 - i : 2 * ng + 1 - i);
+    if (i > hi) return bc == 2 ? i - n0 : (face ? 2 * (n0 + ng) + 2 - i : 2 * (n0 + ng) + 1 - i);
+    return i;
+}
+
+__device__ __forceinline__ int ghost_coord(int g, int ng, int own) { return g < ng ? g + 1 : ng + own + (g - ng) + 1; }
+
+__global__ void __launch_bounds__(kBlock) k_fill(FillSet fs, int ng) {
+    const FillArr A = fs.arr[blockIdx.y];
+    const long t = gtid();
+    int own[3], len[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        len[a] = A.n[a];
+        own[a] = fs.n0[a] + (A.face == a + 1 ? 1 : 0);
+    }
+    const int g2 = 2 * ng;
+    const long c1 = (long)g2 * len[1] * len[2];
+    const long c2 = (long)own[0] * g2 * len[2];
+    const long c3 = (long)own[0] * own[1] * g2;
+    int i, j, k;
+    if (t < c1) {
+        long r = t / g2;
+        i = ghost_coord((int)(t - r * g2), ng, own[0]);
+        j = (int)(r % len[1]) + 1;
+        k = (int)(r / len[1]) + 1;
+    } else if (t < c1 + c2) {
+        long u = t - c1;
+        long r = u / own[0];
+        i = ng + 1 + (int)(u - r * own[0]);
+        j = ghost_coord((int)(r % g2), ng, own[1]);
+        k = (int)(r / g2) + 1;
+    } else if (t < c1 + c2 + c3) {
+        long u = t - c1 - c2;
+        long r = u / own[0];
+        i = ng + 1 + (int)(u - r * own[0]);
+        j = ng + 1 + (int)(r % own[1]);
+        k = ghost_coord((int)(r / own[1]), ng, own[2]);
+    } else {
+        return;
+    }
+    const int si = fill_map(i, fs.n0[0], ng, A.face == 1, fs.bc[0]);
+    const int sj = fill_map(j, fs.n0[1], ng, A.face == 2, fs.bc[1]);
+    const int sk = fill_map(k, fs.n0[2], ng, A.face == 3, fs.bc[2]);
+    A.a[ix(len[0], len[1], i, j, k)] = A.a[ix(len[0], len[1], si, sj, sk)];
+}
+
+// ------------------------------------------------------------------------
+// get_bxyz (mhd3d3.f90:2118-2128) over the ghost-inclusive block, plus the
+// EMF's fast speed vf=|v|+|B|/sqrt(rho) (mhd3d3.f90:1766) of the same state
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_bxyz(Dims d, const double* __restrict__ bi, const double* __restrict__ bj,
+                                                const double* __restrict__ bk, const double* __restrict__ rho,
+                                                const double* __restrict__ vx, const double* __restrict__ vy,
+                                                const double* __restrict__ vz, double* __restrict__ bx,
+                                                double* __restrict__ by, double* __restrict__ bz,
+                                                double* __restrict__ vf, long ncell) {
+    const long t = gtid();
+    if (t >= ncell) return;
+    int i, j, k;
+    unflat(t, d.nx, d.ny, i, j, k);
+    const double x = .5 * (bi[ix(d.nx + 1, d.ny, i + 1, j, k)] + bi[ix(d.nx + 1, d.ny, i, j, k)]);
+    const double y = .5 * (bj[ix(d.nx, d.ny + 1, i, j + 1, k)] + bj[ix(d.nx, d.ny + 1, i, j, k)]);
+    const double z = .5 * (bk[t + (ptrdiff_t)d.nx * d.ny] + bk[t]);
+    bx[t] = x;
+    by[t] = y;
+    bz[t] = z;
+    if (vf) {
+        const double u = vx[t], v = vy[t], w = vz[t];
+        vf[t] = sqrt(u * u + v * v + w * w) + sqrt((x * x + y * y + z * z) / rho[t]);
+    }
+}
+
+// vf of an arbitrary state (get_eijk_LFM on a given state, mhd3d3.f90:1766)
+__global__ void __launch_bounds__(kBlock) k_vf(Prim<double> q, double* __restrict__ vf, long ncell) {
+    const long t = gtid();
+    if (t >= ncell) return;
+    const double u = q.vx[t], v = q.vy[t], w = q.vz[t], x = q.bx[t], y = q.by[t], z = q.bz[t];
+    vf[t] = sqrt(u * u + v * v + w * w) + sqrt((x * x + y * y + z * z) / q.rho[t]);
+}
+
+// ------------------------------------------------------------------------
+// CFL timestep: get_dt (mhd3d3.f90:1608-1639), min over the ghost-inclusive
+// block.  Pass 1: per-workgroup min (wavefront shuffles + LDS); pass 2: one
+// workgroup reduces the partials.  Deterministic, no atomics, no memset.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ double block_min(double v) {
+    __shared__ double red[kBlock / 64];
+    v = wave_min(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = red[q] < m ? red[q] : m;
+        red[0] = m;
+    }
+    __syncthreads();
+    return red[0];
+}
+
+__global__ void __launch_bounds__(kBlock) k_dt_partial(Dims d, double gam, double cfl, Prim<double> q,
+                                                      const double* __restrict__ dx, const double* __restrict__ dy,
+                                                      const double* __restrict__ dz, double* __restrict__ part,
+                                                      long ncell) {
+    const long t = gtid();
+    double dt = __builtin_huge_val();
+    if (t < ncell) {
+        int i, j, k;
+        unflat(t, d.nx, d.ny, i, j, k);
+        const double u = q.vx[t], v = q.vy[t], w = q.vz[t], x = q.bx[t], y = q.by[t], z = q.bz[t];
+        const double r = q.rho[t];
+        const double vv = sqrt(u * u + v * v + w * w);
+        const double btot = sqrt(x * x + y * y + z * z);
+        const double valf = btot / sqrt(r);
+        const double vs = sqrt(gam * q.p[t] / r);
+        const double vcfl = vv + sqrt(valf * valf + vs * vs);
+        dt = cfl / (vcfl / dx[i - 1] + vcfl / dy[j - 1] + vcfl / dz[k - 1]);
+    }
+    dt = block_min(dt);
+    if (threadIdx.x == 0) part[blockIdx.x] = dt;
+}
+
+// out[0] = min(part); if tau != null also tau[0] = out[0] (AB: tau from CFL)
+__global__ void __launch_bounds__(1024) k_dt_final(const double* __restrict__ part, long npart,
+                                                   double* __restrict__ out, double* __restrict__ tau) {
+    __shared__ double red[16];
+    double m = __builtin_huge_val();
+    for (long q = threadIdx.x; q < npart; q += blockDim.x) m = part[q] < m ? part[q] : m;
+    m = wave_min(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) r = red[q] < r ? red[q] : r;
+        out[0] = r;
+        if (tau) tau[0] = r;
+    }
+}
+
+__global__ void k_set2(double* __restrict__ p, double a, double b) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        p[0] = a;
+        p[1] = b;
+    }
+}
+template <int Z = 0>
+__global__ void k_zero_int(int* __restrict__ p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = Z;
+}
+__global__ void k_tau0_from_tau(double* __restrict__ p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) p[1] = p[0];
+}
+
+// ------------------------------------------------------------------------
+// advance_previous (mhd3d3.f90:3794-3819), RK flavour, full arrays
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_prev_rk(double gm1, Prim<double> q, double* __restrict__ rhop,
+                                                   double* __restrict__ mxp, double* __restrict__ myp,
+                                                   double* __restrict__ mzp, double* __restrict__ engp,
+                                                   const double* __restrict__ bi, const double* __restrict__ bj,
+                                                   const double* __restrict__ bk, double* __restrict__ bip,
+                                                   double* __restrict__ bjp, double* __restrict__ bkp, long ncell,
+                                                   long nbi, long nbj, long nbk) {
+    const long t = gtid();
+    if (t < ncell) {
+        const double r = q.rho[t], u = q.vx[t], v = q.vy[t], w = q.vz[t];
+        rhop[t] = r;
+        mxp[t] = r * u;
+        myp[t] = r * v;
+        mzp[t] = r * w;
+        engp[t] = .5 * r * (u * u + v * v + w * w) + q.p[t] / gm1;
+    }
+    if (t < nbi) bip[t] = bi[t];
+    if (t < nbj) bjp[t] = bj[t];
+    if (t < nbk) bkp[t] = bk[t];
+}
+
+// ------------------------------------------------------------------------
+// Adams-Bashforth predictor (mhd3d3.f90:3821-3840) fused with
+// advance_previous (AB flavour) and the h-state's vf:
+//   xh = x + tau/tau0/2*(x - xp);  xp = x      for the 11 state arrays
+// ------------------------------------------------------------------------
+struct AB11 {
+    double* x[11];   // rho vx vy vz p bx by bz bi bj bk  (read)
+    double* xp[11];  // previous (read, then overwritten by x)
+    double* xh[11];  // extrapolated (written)
+};
+
+__global__ void __launch_bounds__(kBlock) k_predict_ab(AB11 s, const double* __restrict__ tau_p,
+                                                      double* __restrict__ vfh, long ncell, long nbi, long nbj,
+                                                      long nbk) {
+    const long t = gtid();
+    const double c = tau_p[0] / tau_p[1] / 2.;
+    if (t < ncell) {
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const double x = s.x[a][t], xp = s.xp[a][t];
+            s.xh[a][t] = x + c * (x - xp);
+            s.xp[a][t] = x;
+        }
+        const double u = s.xh[1][t], v = s.xh[2][t], w = s.xh[3][t];
+        const double bx = s.xh[5][t], by = s.xh[6][t], bz = s.xh[7][t];
+        vfh[t] = sqrt(u * u + v * v + w * w) + sqrt((bx * bx + by * by + bz * bz) / s.xh[0][t]);
+    }
+    const long nf[3] = {nbi, nbj, nbk};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (t < nf[a]) {
+            const double x = s.x[8 + a][t], xp = s.xp[8 + a][t];
+            s.xh[8 + a][t] = x + c * (x - xp);
+            s.xp[8 + a][t] = x;
+        }
+}
+
+// ------------------------------------------------------------------------
+// state diagnostics (diag_rho_neg, diag_overflow, mhd3d3.f90:3143-3160) plus
+// a non-positive pressure check: flags |= 1 (rho<0 anywhere), 2 (NaN in
+// interior rho), 4 (p<=0 or NaN in interior p)
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_check(Dims d, const double* __restrict__ rho,
+                                                 const double* __restrict__ p, int* __restrict__ flags,
+                                                 long ncell) {
+    const long t = gtid();
+    int f = 0;
+    if (t < ncell) {
+        int i, j, k;
+        unflat(t, d.nx, d.ny, i, j, k);
+        const double r = rho[t];
+        if (r < 0.) f |= 1;
+        const bool in = i > d.ng && i <= d.ng + d.nx0 && j > d.ng && j <= d.ng + d.ny0 && k > d.ng &&
+                        k <= d.ng + d.nz0;
+        if (in && r != r) f |= 2;
+        const double pp = p[t];
+        if (in && !(pp > 0.)) f |= 4;
+    }
+    // wave-level OR, then one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) f |= __shfl_xor(f, o, 64);
+    if ((threadIdx.x & 63) == 0 && f) atomicOr(flags, f);
+}
+
+}  // namespace mhd

@@ -1,0 +1,246 @@
+"""ctypes binding of libmhd_gpu.so (C-ABI in include/mhd_gpu.h).
+
+The Python mirror of the Fortran driver's view of the library: a ``Solver``
+owns one context (one block on one GPU) and exposes the reference's per-step
+routines by their reference names (get_dt, fill_hydro_mpi+fill_mag_mpi,
+get_bxyz, get_flux_stress, get_eijk_LFM, step).  Errors raise ``MhdError``
+carrying the library's status code; the library is never replaced by a
+fallback: if libmhd_gpu.so is missing this module raises on import of the
+binding.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(PKG, "libmhd_gpu.so")
+
+MHD_OK, MHD_ERR_HIP, MHD_ERR_PARAM, MHD_ERR_STATE, MHD_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
+BC = {"even": 0, "periodic": 2}
+PRECISION = {"fp64": 0, "mixed": 1}
+KCLASS = {"flux": 0, "emf": 1, "update": 2, "ct": 3, "fill": 4, "bxyz": 5, "dt": 6, "prev": 7, "surrogate": 8}
+FETCH_EI, FETCH_EJ, FETCH_EK, FETCH_VF = 24, 25, 26, 27
+
+
+class MhdError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        super().__init__("%s failed: %s (code %d)" % (where, _errstr(code), code))
+
+
+class Params(C.Structure):
+    _fields_ = [("nx0", C.c_int), ("ny0", C.c_int), ("nz0", C.c_int), ("ng", C.c_int),
+                ("gam", C.c_double), ("cfl", C.c_double), ("pdmb", C.c_double), ("eps", C.c_double),
+                ("alpha", C.c_double), ("beta", C.c_double), ("reconop", C.c_int), ("stepop", C.c_int),
+                ("bc", C.c_int * 3), ("precision", C.c_int), ("surrogate", C.c_int), ("device", C.c_int)]
+
+
+_D = C.POINTER(C.c_double)
+_lib = None
+
+# every entry point of include/mhd_gpu.h with its ctypes signature
+SIGNATURES = {
+    "mhd_default_params": (None, [C.POINTER(Params)]),
+    "mhd_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(Params)]),
+    "mhd_destroy": (None, [C.c_void_p]),
+    "mhd_error_string": (C.c_char_p, [C.c_int]),
+    "mhd_set_geometry": (C.c_int, [C.c_void_p, _D, _D, _D]),
+    "mhd_upload_state": (C.c_int, [C.c_void_p] + [_D] * 8),
+    "mhd_download_state": (C.c_int, [C.c_void_p] + [_D] * 11),
+    "mhd_initialize": (C.c_int, [C.c_void_p, _D]),
+    "mhd_step": (C.c_int, [C.c_void_p, C.c_int, _D, _D]),
+    "mhd_get_dt": (C.c_int, [C.c_void_p, _D]),
+    "mhd_fill_ghosts": (C.c_int, [C.c_void_p]),
+    "mhd_get_bxyz": (C.c_int, [C.c_void_p]),
+    "mhd_flux_stress": (C.c_int, [C.c_void_p]),
+    "mhd_eijk_lfm": (C.c_int, [C.c_void_p]),
+    "mhd_fetch": (C.c_int, [C.c_void_p, C.c_int, _D]),
+    "mhd_reconstruct": (C.c_int, [C.c_void_p, C.c_int, _D, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  _D, _D]),
+    "mhd_check_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "mhd_synchronize": (C.c_int, [C.c_void_p]),
+    "mhd_device_pointer": (C.c_void_p, [C.c_void_p, C.c_int]),
+    "mhd_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "mhd_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]),
+}
+
+
+def lib():
+    """load libmhd_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise OSError("libmhd_gpu.so not built: run __graft_entry__.build() or "
+                          "python mhd3d_mpi_fortran_amd/build.py")
+        L = C.CDLL(LIBPATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _errstr(code):
+    try:
+        return lib().mhd_error_string(code).decode()
+    except OSError:
+        return "?"
+
+
+def _ck(rc, where):
+    if rc != MHD_OK:
+        raise MhdError(rc, where)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def make_params(nx0, ny0, nz0, ng=4, reconop=1, stepop=2, bc="periodic", precision="fp64", surrogate=0,
+                gam=5. / 3., cfl=.3, pdmb=4., eps=1e-8, alpha=2., beta=4., device=0):
+    p = Params()
+    lib().mhd_default_params(C.byref(p))
+    p.nx0, p.ny0, p.nz0, p.ng = nx0, ny0, nz0, ng
+    p.gam, p.cfl, p.pdmb, p.eps, p.alpha, p.beta = gam, cfl, pdmb, eps, alpha, beta
+    p.reconop, p.stepop = reconop, stepop
+    b = BC[bc] if isinstance(bc, str) else bc
+    bcs = b if isinstance(b, (tuple, list)) else (b, b, b)
+    for q in range(3):
+        p.bc[q] = BC[bcs[q]] if isinstance(bcs[q], str) else bcs[q]
+    p.precision = PRECISION[precision] if isinstance(precision, str) else precision
+    p.surrogate = int(surrogate)
+    p.device = device
+    return p
+
+
+class Solver:
+    """one block of the MHD step on one GPU (a libmhd_gpu context)."""
+
+    STATE = ("rho", "vx", "vy", "vz", "p", "bi", "bj", "bk")
+
+    def __init__(self, params, grid):
+        self.params = params
+        self.grid = grid
+        h = C.c_void_p()
+        _ck(lib().mhd_create(C.byref(h), C.byref(params)), "mhd_create")
+        self.h = h
+        dx = np.ascontiguousarray(grid.dx)
+        dy = np.ascontiguousarray(grid.dy)
+        dz = np.ascontiguousarray(grid.dz)
+        _ck(lib().mhd_set_geometry(self.h, _p(dx), _p(dy), _p(dz)), "mhd_set_geometry")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mhd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---------------------------------------------------------------- state
+    def _shapes(self):
+        g = self.grid
+        cs = (g.nx, g.ny, g.nz)
+        return dict(rho=cs, vx=cs, vy=cs, vz=cs, p=cs, bx=cs, by=cs, bz=cs, bi=(g.nx + 1, g.ny, g.nz),
+                    bj=(g.nx, g.ny + 1, g.nz), bk=(g.nx, g.ny, g.nz + 1))
+
+    def upload(self, st):
+        arrs = []
+        for k, shp in zip(self.STATE, [self._shapes()[k] for k in self.STATE]):
+            a = np.asfortranarray(st[k], dtype=np.float64)
+            if a.shape != shp:
+                raise ValueError("%s has shape %s, expected %s" % (k, a.shape, shp))
+            arrs.append(a)
+        _ck(lib().mhd_upload_state(self.h, *[_p(a) for a in arrs]), "mhd_upload_state")
+
+    def download(self):
+        shp = self._shapes()
+        out = {k: np.zeros(s, order="F") for k, s in shp.items()}
+        order = ("rho", "vx", "vy", "vz", "p", "bi", "bj", "bk", "bx", "by", "bz")
+        _ck(lib().mhd_download_state(self.h, *[_p(out[k]) for k in order]), "mhd_download_state")
+        return out
+
+    # ------------------------------------------------------- reference routines
+    def initialize(self, tau):
+        t = C.c_double(tau)
+        _ck(lib().mhd_initialize(self.h, C.byref(t)), "mhd_initialize")
+        return t.value
+
+    def step(self, nsteps=1, tau=0.):
+        t = C.c_double(tau)
+        tc = C.c_double(0.)
+        _ck(lib().mhd_step(self.h, int(nsteps), C.byref(t), C.byref(tc)), "mhd_step")
+        return t.value, tc.value
+
+    def get_dt(self):
+        t = C.c_double(0.)
+        _ck(lib().mhd_get_dt(self.h, C.byref(t)), "mhd_get_dt")
+        return t.value
+
+    def fill(self):
+        _ck(lib().mhd_fill_ghosts(self.h), "mhd_fill_ghosts")
+
+    def bxyz(self):
+        _ck(lib().mhd_get_bxyz(self.h), "mhd_get_bxyz")
+
+    def flux_stress(self):
+        _ck(lib().mhd_flux_stress(self.h), "mhd_flux_stress")
+        g = self.grid
+        out = []
+        for d in range(3):
+            shp = (g.nx0 + (d == 0), g.ny0 + (d == 1), g.nz0 + (d == 2))
+            for q in range(8):
+                a = np.zeros(shp, order="F")
+                _ck(lib().mhd_fetch(self.h, 8 * d + q, _p(a)), "mhd_fetch")
+                out.append(a)
+        return out
+
+    def eijk(self):
+        _ck(lib().mhd_eijk_lfm(self.h), "mhd_eijk_lfm")
+        g = self.grid
+        shp = {FETCH_EI: (g.nx0, g.ny0 + 1, g.nz0 + 1), FETCH_EJ: (g.nx0 + 1, g.ny0, g.nz0 + 1),
+               FETCH_EK: (g.nx0 + 1, g.ny0 + 1, g.nz0)}
+        out = {}
+        for w, name in ((FETCH_EI, "ei"), (FETCH_EJ, "ej"), (FETCH_EK, "ek")):
+            a = np.zeros(shp[w], order="F")
+            _ck(lib().mhd_fetch(self.h, w, _p(a)), "mhd_fetch")
+            out[name] = a
+        return out
+
+    def check_state(self):
+        f = C.c_int(0)
+        rc = lib().mhd_check_state(self.h, C.byref(f))
+        if rc not in (MHD_OK, MHD_ERR_STATE):
+            raise MhdError(rc, "mhd_check_state")
+        return f.value
+
+    def synchronize(self):
+        _ck(lib().mhd_synchronize(self.h), "mhd_synchronize")
+
+    def profile(self, on=True):
+        _ck(lib().mhd_profile(self.h, 1 if on else 0), "mhd_profile")
+
+    def profile_read(self, kclass):
+        n = C.c_int(0)
+        ms = C.c_double(0.)
+        _ck(lib().mhd_profile_read(self.h, KCLASS[kclass] if isinstance(kclass, str) else kclass, C.byref(n),
+                                   C.byref(ms)), "mhd_profile_read")
+        return n.value, ms.value
+
+
+def reconstruct(solver, f, lshape, direction):
+    """reconstruct_3d of a host array through the library."""
+    f = np.asfortranarray(f, dtype=np.float64)
+    fl = np.zeros(lshape, order="F")
+    fr = np.zeros(lshape, order="F")
+    _ck(lib().mhd_reconstruct(solver.h, direction, _p(f), f.shape[0], f.shape[1], f.shape[2], lshape[0], lshape[1],
+                              lshape[2], _p(fl), _p(fr)), "mhd_reconstruct")
+    return fl, fr
