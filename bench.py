@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""Benchmark: zone-updates/s of the ideal-MHD step on one MI355X.
+
+Workload (largest single-GPU configuration of the benchmark set): a 256^3
+periodic box, gamma=5/3, Adams-Bashforth + constrained transport, PDM
+reconstruction, seeded blast wave (4 centres, p=100 inside r<0.1, p=0.1
+outside, rho=1, B=(1,1,0)/sqrt(2)).  Grid: uniform Cartesian (the
+reference's tensor-product geometry).
+
+Two runs from the same initial state:
+  * plain: binary64 everywhere (parity-checked against the CPU oracle);
+  * fast : fp32 reconstruction/flux/EMF with binary64 conserved
+           accumulation and CT face update, plus the learned face
+           reconstruction when trained weights are available.
+`value` is the fast run's whole-job zone-updates/s; the plain run's figure,
+the speed-up and the density L1 difference between the two are reported
+beside it.  A zone-update is one interior cell advanced one full step.
+
+cpu_baseline: the reference solver (oracle/_ref, built from mhd3d3.f90) on
+the host cores with MPI, on a bounded sample of the same workload, timed by
+its own step-loop timer; falls back to the C oracle when the reference
+build or MPI is unavailable.
+"""
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from mhd3d_mpi_fortran_amd import gpu, ics  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic (compulsory) bytes per output point of each kernel class
+# at binary64 state: inputs read once + outputs written once
+#   emf  : v1, v2, vf (cells) + b1, b2 (faces) read, one edge EMF written
+#   flux : 8 primitives read, 8 face products written
+ALG_BYTES = {"emf": {"fp64": 5 * 8 + 8, "mixed": 5 * 8 + 4},
+             "flux": {"fp64": 8 * 8 + 8 * 8, "mixed": 8 * 8 + 8 * 4}}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(n):
+    g = ics.Grid(n, n, n, ng=4)
+    st = ics.initial_state(g, "blast", seed=4, radius=.1, p_in=100., p_out=.1)
+    return g, st
+
+
+def run_gpu(g, st, n, precision, surrogate, warmup, steps):
+    s = gpu.Solver(gpu.make_params(n, n, n, ng=4, reconop=1, stepop=1, bc="periodic", precision=precision,
+                                   surrogate=surrogate), g)
+    s.upload(st)
+    tau = s.initialize(1e-4)
+    if warmup:
+        tau, _ = s.step(warmup, tau)
+    s.synchronize()
+    s.profile(True)
+    t0 = time.perf_counter()
+    tau, tcfl = s.step(steps, tau)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    prof = {k: s.profile_read(k) for k in gpu.KCLASS}
+    s.profile(False)
+    flags = s.check_state()
+    st_out = s.download()
+    s.close()
+    units = {"emf": [n * (n + 1) * (n + 1), (n + 1) * n * (n + 1), (n + 1) * (n + 1) * n],
+             "flux": [(n + 1) * n * n, n * (n + 1) * n, n * n * (n + 1)]}
+    return dict(wall=wall, tau=tau, prof=prof, flags=flags, state=st_out, units=units)
+
+
+def l1_rel(a, b, g):
+    ra, rb = g.interior(a["rho"]), g.interior(b["rho"])
+    return float(np.abs(ra - rb).sum() / np.abs(rb).sum())
+
+
+def max_divb(st, g):
+    bi, bj, bk = g.interior(st["bi"], 1), g.interior(st["bj"], 2), g.interior(st["bk"], 3)
+    ng = g.ng
+    dx = g.dx[ng:ng + g.nx0][:, None, None]
+    dy = g.dy[ng:ng + g.ny0][None, :, None]
+    dz = g.dz[ng:ng + g.nz0][None, None, :]
+    div = (bi[1:] - bi[:-1]) / dx + (bj[:, 1:] - bj[:, :-1]) / dy + (bk[:, :, 1:] - bk[:, :, :-1]) / dz
+    return float(np.abs(div).max())
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_reference(g, st, n, nsteps, ranks):
+    """time the reference Fortran (oracle/_ref) with MPI; None if unavailable."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "mhd3d3_ref_periodic.x")
+    mpiexec = "/opt/conda/bin/mpiexec"
+    if not (os.path.exists(exe) and os.path.exists(mpiexec)):
+        return None
+    px = {16: (4, 2, 2), 8: (2, 2, 2)}[ranks]
+    run = tempfile.mkdtemp(prefix="mhdref_bench_")
+    try:
+        with open(os.path.join(run, "ic.bin"), "wb") as fh:
+            for v in ("rho", "vx", "vy", "vz", "p", "bi", "bj", "bk"):
+                face = {"bi": 1, "bj": 2, "bk": 3}.get(v, 0)
+                fh.write(np.asfortranarray(g.interior(st[v], face)).tobytes(order="F"))
+        deck = (" &dom\n xmin=0., xmax=1., ymin=0., ymax=1., zmin=0., zmax=1.\n/\n"
+                " &grid\n nx0G=%d, ny0G=%d, nz0G=%d, xprocs=%d, yprocs=%d, zprocs=%d\n/\n"
+                " &comset\n nord=8, reconop=1, stepop=1, outop=0\n/\n"
+                " &list\n nts=1, tau=1.e-4, nframes=%d, nrst=0, nfdump=10\n/\n"
+                " &free\n cfl=.3, pdmb=4., eps=1.e-8, alpha=2., beta=4.\n/\n") % (n, n, n, px[0], px[1], px[2],
+                                                                                  nsteps)
+        with open(os.path.join(run, "mhd3d3.in"), "w") as fh:
+            fh.write(deck)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        r = subprocess.run([mpiexec, "-n", str(ranks), exe], cwd=run, capture_output=True, text=True,
+                           timeout=900, env=env)
+        if r.returncode != 0:
+            log("reference run failed:", r.stdout[-500:], r.stderr[-500:])
+            return None
+        for line in r.stdout.splitlines():
+            if "STEPLOOP_SECONDS" in line:
+                t = float(line.split()[-1])
+                return dict(value=n ** 3 * nsteps / t, unit="zone-updates/s", cores=ranks, kind="reference",
+                            sample="%d^3 blast wave, PDM AB2, %d steps, mpiexec -n %d (%dx%dx%d), step loop "
+                                   "timed by MPI_Wtime" % (n, nsteps, ranks, px[0], px[1], px[2]))
+        return None
+    except (subprocess.TimeoutExpired, OSError) as e:
+        log("reference run failed:", e)
+        return None
+    finally:
+        shutil.rmtree(run, ignore_errors=True)
+
+
+def cpu_port(g, st, n, nsteps, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    po.lib().orc_set_threads(threads)
+    o = po.Oracle(po.make_params(n, n, n, ng=4, reconop=1, stepop=1, bc="periodic"), g)
+    o.set_state(st)
+    tau = o.initialize(1e-4)
+    t0 = time.perf_counter()
+    o.step(nsteps, tau)
+    t = time.perf_counter() - t0
+    return dict(value=n ** 3 * nsteps / t, unit="zone-updates/s", cores=threads, kind="port",
+                sample="%d^3 blast wave, PDM AB2, %d steps, C oracle with %d OpenMP threads" % (n, nsteps, threads))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--only", choices=["plain", "fast", "both"], default="both")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the MPI decomposition of the reference is replaced by a single "
+              "whole-domain block; --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    n = args.n
+    g, st = workload(n)
+    sur = 1 if os.path.exists(os.path.join(ROOT, "mhd3d_mpi_fortran_amd", "weights", "surrogate_pdm.bin")) else 0
+    res = {}
+    if args.only in ("plain", "both"):
+        log("plain fp64 run ...")
+        res["plain"] = run_gpu(g, st, n, "fp64", 0, args.warmup, args.steps)
+    if args.only in ("fast", "both"):
+        log("fast run (mixed precision%s) ..." % (" + learned reconstruction" if sur else ""))
+        res["fast"] = run_gpu(g, st, n, "mixed", sur, args.warmup, args.steps)
+    head = res.get("fast") or res["plain"]
+    head_name = "fast" if "fast" in res else "plain"
+    prec = "mixed" if head_name == "fast" else "fp64"
+    K = args.steps
+    value = n ** 3 * K / head["wall"]
+
+    # dominant kernel of the headline run, timed by HIP events on the
+    # library's stream over the timed region
+    kcls = max(("emf", "flux"), key=lambda k: head["prof"][k][1])
+    launches, ms = head["prof"][kcls]
+    avg_s = ms / launches / 1e3
+    units = sum(head["units"][kcls]) / 3.
+    alg = ALG_BYTES[kcls][prec] * units
+    achieved = alg / avg_s / 1e9
+
+    out = {
+        "metric": "zone-updates/s on 1 MI355X: learned+FP32 vs plain FP64 at equal L1 error",
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": head["wall"] / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 stencils + f64 accumulate" if prec == "mixed" else "f64",
+        "data": "synthetic (seeded blast wave)",
+        "config": {"workload": "%d^3 periodic blast wave, AB2 + CT, PDM, gamma=5/3" % n,
+                   "mode": "mixed precision" + (" + learned reconstruction" if (sur and prec == "mixed") else ""),
+                   "grid": "uniform Cartesian", "parallelism": "single GPU, whole-domain block"},
+        "roofline": {"bound": "hbm", "kernel": kcls, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": avg_s * 1e3, "alg_bytes_per_launch": alg},
+    }
+    if "plain" in res and "fast" in res:
+        pv = n ** 3 * K / res["plain"]["wall"]
+        out["plain_fp64"] = {"value": pv, "ms_per_step": res["plain"]["wall"] / K * 1e3,
+                             "max_divB": max_divb(res["plain"]["state"], g)}
+        out["speedup_vs_plain_fp64"] = value / pv
+        out["density_L1_vs_plain_fp64"] = l1_rel(res["fast"]["state"], res["plain"]["state"], g)
+        out["max_divB"] = max_divb(res["fast"]["state"], g)
+    out["kernel_ms_per_step"] = {k: v[1] / K for k, v in head["prof"].items() if v[0]}
+    if not args.no_cpu_baseline:
+        cores = host_cores()
+        ranks = 16 if cores >= 16 else 8
+        log("cpu baseline: reference with %d MPI ranks ..." % ranks)
+        cb = cpu_reference(g, st, n, args.cpu_steps, ranks)
+        if cb is None:
+            m = 64
+            g2, st2 = workload(m)
+            cb = cpu_port(g2, st2, m, 2, cores)
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

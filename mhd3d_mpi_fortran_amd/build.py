@@ -70,7 +70,7 @@ def build_driver(verbose=False):
     if not (os.path.exists(mod) and os.path.exists(drv)):
         return None
     tmp = DRIVER + ".tmp"
-    _run([FLANG, "-O2", "-J", BUILD, mod, drv, "-o", tmp, "-L" + PKG, "-lmhd_gpu", "-Wl,-rpath,$ORIGIN",
+    _run([FLANG, "-O2", "-ffp-contract=off", "-J", BUILD, mod, drv, "-o", tmp, "-L" + PKG, "-lmhd_gpu", "-Wl,-rpath,$ORIGIN",
           "-Wl,-rpath," + PKG])
     os.replace(tmp, DRIVER)
     if verbose:

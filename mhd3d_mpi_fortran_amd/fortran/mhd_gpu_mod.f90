@@ -1,0 +1,134 @@
+!-----------------------------------------------------------------------
+! mhd_gpu_mod.f90 -- ISO_C_BINDING interface to libmhd_gpu.so
+! (C declarations in include/mhd_gpu.h).
+!
+! The reference's per-step routines (step, get_dt, fill_hydro_mpi +
+! fill_mag_mpi, get_bxyz, get_flux_stress, get_eijk_LFM; mhd3d3.f90) map
+! one-to-one onto these bindings.  Arrays are passed as assumed-size
+! real(c_double) dummies in the reference layout (column-major, x fastest,
+! ghost-inclusive).
+!-----------------------------------------------------------------------
+module mhd_gpu
+   use, intrinsic :: iso_c_binding
+   implicit none
+
+   integer(c_int), parameter :: MHD_OK = 0, MHD_ERR_HIP = 1, MHD_ERR_PARAM = 2, &
+                                MHD_ERR_STATE = 3, MHD_ERR_UNSUPPORTED = 4
+   integer(c_int), parameter :: MHD_BC_EVEN = 0, MHD_BC_PERIODIC = 2
+   integer(c_int), parameter :: MHD_FP64 = 0, MHD_MIXED = 1
+
+   type, bind(C) :: mhd_params
+      integer(c_int) :: nx0, ny0, nz0, ng
+      real(c_double) :: gam, cfl, pdmb, eps, alpha, beta
+      integer(c_int) :: reconop, stepop
+      integer(c_int) :: bc(3)
+      integer(c_int) :: precision, surrogate, device
+   end type mhd_params
+
+   interface
+      subroutine mhd_default_params(p) bind(C, name='mhd_default_params')
+         import :: mhd_params
+         type(mhd_params), intent(out) :: p
+      end subroutine
+      integer(c_int) function mhd_create(ctx, p) bind(C, name='mhd_create')
+         import :: c_int, c_ptr, mhd_params
+         type(c_ptr), intent(out) :: ctx
+         type(mhd_params), intent(in) :: p
+      end function
+      subroutine mhd_destroy(ctx) bind(C, name='mhd_destroy')
+         import :: c_ptr
+         type(c_ptr), value :: ctx
+      end subroutine
+      type(c_ptr) function mhd_error_string(code) bind(C, name='mhd_error_string')
+         import :: c_ptr, c_int
+         integer(c_int), value :: code
+      end function
+      integer(c_int) function mhd_set_geometry(ctx, dx, dy, dz) bind(C, name='mhd_set_geometry')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: dx(*), dy(*), dz(*)
+      end function
+      integer(c_int) function mhd_upload_state(ctx, rho, vx, vy, vz, p, bi, bj, bk) &
+            bind(C, name='mhd_upload_state')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: rho(*), vx(*), vy(*), vz(*), p(*), bi(*), bj(*), bk(*)
+      end function
+      integer(c_int) function mhd_download_state(ctx, rho, vx, vy, vz, p, bi, bj, bk, bx, by, bz) &
+            bind(C, name='mhd_download_state')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: rho(*), vx(*), vy(*), vz(*), p(*), bi(*), bj(*), bk(*), &
+                                        bx(*), by(*), bz(*)
+      end function
+      integer(c_int) function mhd_initialize(ctx, tau) bind(C, name='mhd_initialize')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(inout) :: tau
+      end function
+      integer(c_int) function mhd_step(ctx, nsteps, tau, tau_cfl) bind(C, name='mhd_step')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nsteps
+         real(c_double), intent(inout) :: tau
+         real(c_double), intent(out) :: tau_cfl
+      end function
+      integer(c_int) function mhd_get_dt(ctx, tau_cfl) bind(C, name='mhd_get_dt')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: tau_cfl
+      end function
+      integer(c_int) function mhd_fill_ghosts(ctx) bind(C, name='mhd_fill_ghosts')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function mhd_get_bxyz(ctx) bind(C, name='mhd_get_bxyz')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function mhd_flux_stress(ctx) bind(C, name='mhd_flux_stress')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function mhd_eijk_lfm(ctx) bind(C, name='mhd_eijk_lfm')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function mhd_fetch(ctx, which, out) bind(C, name='mhd_fetch')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: which
+         real(c_double), intent(out) :: out(*)
+      end function
+      integer(c_int) function mhd_check_state(ctx, flags) bind(C, name='mhd_check_state')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), intent(out) :: flags
+      end function
+      integer(c_int) function mhd_synchronize(ctx) bind(C, name='mhd_synchronize')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+   end interface
+
+contains
+
+   ! stop with the library's message on a failed call (the reference stops
+   ! with a message on every fatal condition, mhd3d3.f90:3139-3155)
+   subroutine mhd_check(rc, where)
+      integer(c_int), intent(in) :: rc
+      character(len=*), intent(in) :: where
+      character(kind=c_char), pointer :: msg(:)
+      integer :: n
+      if (rc == MHD_OK) return
+      call c_f_pointer(mhd_error_string(rc), msg, [128])
+      n = 0
+      do while (n < 128)
+         if (msg(n+1) == c_null_char) exit
+         n = n + 1
+      enddo
+      print '(A,A,A,I0,A,128A1)', 'mhd_gpu: ', where, ' failed (code ', rc, '): ', msg(1:n)
+      stop 1
+   end subroutine mhd_check
+
+end module mhd_gpu
