@@ -38,12 +38,13 @@ sys.path.insert(0, ROOT)
 from mhd3d_mpi_fortran_amd import gpu, ics  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic (compulsory) bytes per output point of each kernel class
-# at binary64 state: inputs read once + outputs written once
+# algorithmic (compulsory) bytes per output point of each kernel class:
+# inputs read once + outputs written once (fp64 mode: binary64 inputs and
+# outputs; mixed mode: fp32 stencil inputs and fp32 products)
 #   emf  : v1, v2, vf (cells) + b1, b2 (faces) read, one edge EMF written
 #   flux : 8 primitives read, 8 face products written
-ALG_BYTES = {"emf": {"fp64": 5 * 8 + 8, "mixed": 5 * 8 + 4},
-             "flux": {"fp64": 8 * 8 + 8 * 8, "mixed": 8 * 8 + 8 * 4}}
+ALG_BYTES = {"emf": {"fp64": 5 * 8 + 8, "mixed": 5 * 4 + 4},
+             "flux": {"fp64": 8 * 8 + 8 * 8, "mixed": 8 * 4 + 8 * 4}}
 
 
 def log(*a):

@@ -87,7 +87,12 @@ int mhd_create(mhd_ctx** out, const mhd_params* p) {
     c->ne[1] = (long)(d.nx0 + 1) * d.ny0 * (d.nz0 + 1);
     c->ne[2] = (long)(d.nx0 + 1) * (d.ny0 + 1) * d.nz0;
     c->ninterior = (long)d.nx0 * d.ny0 * d.nz0;
-    c->ndtpart = (c->ncell + kBlock - 1) / kBlock;
+    long nmax = c->ncell;
+    nmax = c->nbi > nmax ? c->nbi : nmax;
+    nmax = c->nbj > nmax ? c->nbj : nmax;
+    nmax = c->nbk > nmax ? c->nbk : nmax;
+    c->nmax = nmax;
+    c->ndtpart = (nmax + kBlock - 1) / kBlock;
     const size_t tf = p->precision == MHD_MIXED ? sizeof(float) : sizeof(double);
 
     // arena layout
@@ -108,13 +113,21 @@ int mhd_create(mhd_ctx** out, const mhd_params* p) {
     plan.push_back({(void**)&c->dtpart, align_up(c->ndtpart * sizeof(double))});
     plan.push_back({(void**)&c->scal, align_up(8 * sizeof(double))});
     plan.push_back({(void**)&c->flags, align_up(64)});
+    const bool mixed = p->precision == MHD_MIXED;
+    if (mixed) {
+        for (int a = 0; a < 12; ++a) {
+            const long n = a < 9 ? c->ncell : (a == 9 ? c->nbi : (a == 10 ? c->nbj : c->nbk));
+            plan.push_back({(void**)&c->sh[a], align_up(n * sizeof(float))});
+        }
+    }
     if (p->stepop == 1) {
         for (int a = 0; a < 11; ++a) {
             const long n = a < 8 ? c->ncell : (a == 8 ? c->nbi : (a == 9 ? c->nbj : c->nbk));
             plan.push_back({(void**)&c->xp[a], align_up(n * sizeof(double))});
-            plan.push_back({(void**)&c->xh[a], align_up(n * sizeof(double))});
+            plan.push_back({(void**)&c->xn[a], align_up(n * sizeof(double))});
+            if (!mixed) plan.push_back({(void**)&c->xh[a], align_up(n * sizeof(double))});
         }
-        plan.push_back({(void**)&c->vfh, align_up(c->ncell * sizeof(double))});
+        if (!mixed) plan.push_back({(void**)&c->vfh, align_up(c->ncell * sizeof(double))});
     } else {
         double** prev[] = {&c->rhop, &c->mxp, &c->myp, &c->mzp, &c->engp};
         for (auto q : prev) cell(q);
@@ -185,6 +198,7 @@ int mhd_upload_state(mhd_ctx* c, const double* rho, const double* vx, const doub
                           c->nbk * sizeof(double)};
     for (int a = 0; a < 8; ++a)
         if (hipMemcpyAsync(dst[a], src[a], nb[a], hipMemcpyHostToDevice, c->s) != hipSuccess) return MHD_ERR_HIP;
+    c->dt_fresh = false;
     return hipStreamSynchronize(c->s) == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
@@ -209,8 +223,23 @@ int mhd_download_state(mhd_ctx* c, double* rho, double* vx, double* vy, double* 
 static Prim<double> cur_state(mhd_ctx* c) {
     return Prim<double>{c->rho, c->vx, c->vy, c->vz, c->pr, c->bx, c->by, c->bz};
 }
-static Prim<double> h_state(mhd_ctx* c) {
-    return Prim<double>{c->xh[0], c->xh[1], c->xh[2], c->xh[3], c->xh[4], c->xh[5], c->xh[6], c->xh[7]};
+static Src<double> cur_src(mhd_ctx* c) {
+    return Src<double>{c->rho, c->vx, c->vy, c->vz, c->pr, c->bx, c->by, c->bz, c->vf, c->bi, c->bj, c->bk};
+}
+static Src<double> h_src(mhd_ctx* c) {
+    return Src<double>{c->xh[0], c->xh[1], c->xh[2], c->xh[3], c->xh[4], c->xh[5],
+                       c->xh[6], c->xh[7], c->vfh,   c->xh[8], c->xh[9], c->xh[10]};
+}
+static Src<float> sh_src(mhd_ctx* c) {
+    float* const* h = c->sh;
+    return Src<float>{h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7], h[8], h[9], h[10], h[11]};
+}
+static bool mixed(const mhd_ctx* c) { return c->p.precision == MHD_MIXED; }
+
+// the 11 state arrays of the current state (AB rotation order)
+static double** state_slot(mhd_ctx* c, int a) {
+    double** t[11] = {&c->rho, &c->vx, &c->vy, &c->vz, &c->pr, &c->bx, &c->by, &c->bz, &c->bi, &c->bj, &c->bk};
+    return t[a];
 }
 
 static void do_fill(mhd_ctx* c) {
@@ -238,67 +267,78 @@ static void do_fill(mhd_ctx* c) {
     for (int a = 0; a < 3; ++a) fs.bc[a] = c->p.bc[a];
     dim3 grid((unsigned)((most + kBlock - 1) / kBlock), 8);
     launch(c, MHD_K_FILL, grid, dim3(kBlock), k_fill, fs, d.ng);
+    c->dt_fresh = false;
 }
 
-static void do_bxyz(mhd_ctx* c) {
-    launch(c, MHD_K_BXYZ, blocks(c->ncell), dim3(kBlock), k_bxyz, c->d, (const double*)c->bi,
-           (const double*)c->bj, (const double*)c->bk, (const double*)c->rho, (const double*)c->vx,
-           (const double*)c->vy, (const double*)c->vz, c->bx, c->by, c->bz, c->vf, c->ncell);
+// get_bxyz on the current state, plus (optionally) vf, the fp32 stencil
+// copies and the CFL partials of the next get_dt
+static void do_bxyz(mhd_ctx* c, bool want_vf, bool want_shadow, bool want_dt) {
+    Shadow sh{};
+    if (want_shadow && mixed(c))
+        sh = Shadow{c->sh[0], c->sh[1], c->sh[2], c->sh[3], c->sh[4],  c->sh[5],
+                    c->sh[6], c->sh[7], c->sh[8], c->sh[9], c->sh[10], c->sh[11]};
+    launch(c, MHD_K_BXYZ, blocks(c->nmax), dim3(kBlock), k_bxyz, c->d, c->p.gam, c->p.cfl,
+           CFaces{c->bi, c->bj, c->bk}, CCells{c->rho, c->vx, c->vy, c->vz, c->pr}, c->bx, c->by, c->bz,
+           want_vf ? c->vf : (double*)nullptr, sh, (const double*)c->dx, (const double*)c->dy,
+           (const double*)c->dz, want_dt ? c->dtpart : (double*)nullptr, c->ncell, c->nbi, c->nbj, c->nbk);
+    c->dt_fresh = want_dt;
 }
 
 // get_dt on the current state; the minimum lands in scal[SC_DTCFL] and, when
 // set_tau, also in scal[SC_TAU] (Adams-Bashforth takes tau from the CFL)
 static void do_dt(mhd_ctx* c, bool set_tau) {
-    launch(c, MHD_K_DT, blocks(c->ncell), dim3(kBlock), k_dt_partial, c->d, c->p.gam, c->p.cfl, cur_state(c),
-           (const double*)c->dx, (const double*)c->dy, (const double*)c->dz, c->dtpart, c->ncell);
-    launch(c, MHD_K_DT, dim3(1), dim3(1024), k_dt_final, (const double*)c->dtpart, c->ndtpart,
-           c->scal + SC_DTCFL, set_tau ? c->scal + SC_TAU : (double*)nullptr);
+    if (!c->dt_fresh) {
+        launch(c, MHD_K_DT, blocks(c->ncell), dim3(kBlock), k_dt_partial, c->d, c->p.gam, c->p.cfl, cur_state(c),
+               (const double*)c->dx, (const double*)c->dy, (const double*)c->dz, c->dtpart, c->ncell);
+        launch(c, MHD_K_DT, dim3(1), dim3(1024), k_dt_final, (const double*)c->dtpart, blocks(c->ncell).x * 1L,
+               c->scal + SC_DTCFL, set_tau ? c->scal + SC_TAU : (double*)nullptr);
+        return;
+    }
+    launch(c, MHD_K_DT, dim3(1), dim3(1024), k_dt_final, (const double*)c->dtpart, c->ndtpart, c->scal + SC_DTCFL,
+           set_tau ? c->scal + SC_TAU : (double*)nullptr);
 }
 
 static void do_prev_rk(mhd_ctx* c) {
-    long n = c->ncell;
-    n = c->nbi > n ? c->nbi : n;
-    n = c->nbj > n ? c->nbj : n;
-    n = c->nbk > n ? c->nbk : n;
-    launch(c, MHD_K_PREV, blocks(n), dim3(kBlock), k_prev_rk, c->p.gam - 1., cur_state(c), c->rhop, c->mxp,
+    launch(c, MHD_K_PREV, blocks(c->nmax), dim3(kBlock), k_prev_rk, c->p.gam - 1., cur_state(c), c->rhop, c->mxp,
            c->myp, c->mzp, c->engp, (const double*)c->bi, (const double*)c->bj, (const double*)c->bk, c->bip,
            c->bjp, c->bkp, c->ncell, c->nbi, c->nbj, c->nbk);
 }
 
+// conserved update + CT from the current state into `out` (the current
+// state itself for RK, the next-state buffers for AB)
 template <typename TF, int NST, int ST>
-static void do_update_t(mhd_ctx* c) {
-    Cells s{c->rho, c->vx, c->vy, c->vz, c->pr};
+static void do_update_t(mhd_ctx* c, double* const* out) {
+    CCells s{c->rho, c->vx, c->vy, c->vz, c->pr};
+    Cells so{out[0], out[1], out[2], out[3], out[4]};
     PrevRK pv{c->rhop, c->mxp, c->myp, c->mzp, c->engp};
     FluxSet<TF> F{(const TF*)c->flux[0], (const TF*)c->flux[1], (const TF*)c->flux[2], c->nf[0], c->nf[1], c->nf[2]};
     launch(c, MHD_K_UPDATE, blocks(c->ninterior), dim3(kBlock), k_update<NST, ST, TF>, c->d, c->p.gam - 1.,
-           (const double*)(c->scal + SC_TAU), s, pv, F, (const double*)c->dx, (const double*)c->dy,
+           (const double*)(c->scal + SC_TAU), s, so, pv, F, (const double*)c->dx, (const double*)c->dy,
            (const double*)c->dz, c->ninterior);
-    Faces b{c->bi, c->bj, c->bk};
+    CFaces b{c->bi, c->bj, c->bk};
+    Faces bo{out[8], out[9], out[10]};
     CFaces bp{c->bip, c->bjp, c->bkp};
     const long nct = (long)(c->d.nx0 + 1) * c->d.ny0 * c->d.nz0 + (long)c->d.nx0 * (c->d.ny0 + 1) * c->d.nz0 +
                      (long)c->d.nx0 * c->d.ny0 * (c->d.nz0 + 1);
-    launch(c, MHD_K_CT, blocks(nct), dim3(kBlock), k_ct<NST, ST, TF>, c->d, (const double*)(c->scal + SC_TAU), b,
+    launch(c, MHD_K_CT, blocks(nct), dim3(kBlock), k_ct<NST, ST, TF>, c->d, (const double*)(c->scal + SC_TAU), b, bo,
            bp, (const TF*)c->emf[0], (const TF*)c->emf[1], (const TF*)c->emf[2], (const double*)c->dx,
            (const double*)c->dy, (const double*)c->dz);
 }
 
 template <int NST, int ST>
-static void do_update(mhd_ctx* c) {
-    if (c->p.precision == MHD_MIXED) do_update_t<float, NST, ST>(c);
-    else do_update_t<double, NST, ST>(c);
+static void do_update(mhd_ctx* c, double* const* out) {
+    if (mixed(c)) do_update_t<float, NST, ST>(c, out);
+    else do_update_t<double, NST, ST>(c, out);
 }
 
-// flux + EMF of a given state (RK: current, AB: extrapolated)
-static void do_ops(mhd_ctx* c, Prim<double> q, const double* bi, const double* bj, const double* bk,
-                   const double* vf) {
-    if (c->p.precision == MHD_MIXED) {
-        launch_emf_fp32(c, q, bi, bj, bk, vf);
-        if (c->p.surrogate) launch_flux_surrogate(c, q);
-        else launch_flux_fp32(c, q);
-    } else {
-        launch_emf_fp64(c, q, bi, bj, bk, vf);
-        launch_flux_fp64(c, q);
-    }
+static void do_ops64(mhd_ctx* c, Src<double> s) {
+    launch_emf_fp64(c, s);
+    launch_flux_fp64(c, s);
+}
+static void do_ops32(mhd_ctx* c) {
+    launch_emf_fp32(c, sh_src(c));
+    if (c->p.surrogate) launch_flux_surrogate(c, sh_src(c));
+    else launch_flux_fp32(c, sh_src(c));
 }
 
 static int sync_status(mhd_ctx* c) {
@@ -320,9 +360,9 @@ int mhd_fill_ghosts(mhd_ctx* c) {
 }
 
 int mhd_get_bxyz(mhd_ctx* c) {
-    if (!c) return MHD_ERR_PARAM;
+    if (!c || !c->have_geometry) return MHD_ERR_PARAM;
     hipSetDevice(c->p.device);
-    do_bxyz(c);
+    do_bxyz(c, true, true, true);
     return sync_status(c);
 }
 
@@ -339,24 +379,37 @@ int mhd_get_dt(mhd_ctx* c, double* tau_cfl) {
 }
 
 int mhd_flux_stress(mhd_ctx* c) {
-    if (!c) return MHD_ERR_PARAM;
+    if (!c || !c->have_geometry) return MHD_ERR_PARAM;
     hipSetDevice(c->p.device);
-    if (c->p.precision == MHD_MIXED) {
-        if (c->p.surrogate) launch_flux_surrogate(c, cur_state(c));
-        else launch_flux_fp32(c, cur_state(c));
+    if (mixed(c)) {
+        do_bxyz(c, false, true, false);
+        if (c->p.surrogate) launch_flux_surrogate(c, sh_src(c));
+        else launch_flux_fp32(c, sh_src(c));
     } else {
-        launch_flux_fp64(c, cur_state(c));
+        launch_flux_fp64(c, cur_src(c));
     }
     return sync_status(c);
 }
 
 int mhd_eijk_lfm(mhd_ctx* c) {
-    if (!c) return MHD_ERR_PARAM;
+    if (!c || !c->have_geometry) return MHD_ERR_PARAM;
     hipSetDevice(c->p.device);
-    launch(c, MHD_K_BXYZ, blocks(c->ncell), dim3(kBlock), k_vf, cur_state(c), c->vf, c->ncell);
-    if (c->p.precision == MHD_MIXED) launch_emf_fp32(c, cur_state(c), c->bi, c->bj, c->bk, c->vf);
-    else launch_emf_fp64(c, cur_state(c), c->bi, c->bj, c->bk, c->vf);
+    if (mixed(c)) {
+        do_bxyz(c, false, true, false);
+        launch_emf_fp32(c, sh_src(c));
+    } else {
+        launch(c, MHD_K_BXYZ, blocks(c->ncell), dim3(kBlock), k_vf, cur_state(c), c->vf, c->ncell);
+        launch_emf_fp64(c, cur_src(c));
+    }
     return sync_status(c);
+}
+
+// vf and fp32 copies are kept only where the next step reads them: RK reads
+// the current state (vf in binary64, the copies in mixed precision), AB
+// reads the extrapolated state written by the predictor
+static void end_of_stage(mhd_ctx* c) {
+    const bool rk = c->p.stepop != 1;
+    do_bxyz(c, rk && !mixed(c), rk && mixed(c), true);
 }
 
 int mhd_initialize(mhd_ctx* c, double* tau) {
@@ -366,17 +419,17 @@ int mhd_initialize(mhd_ctx* c, double* tau) {
     c->tau0_host = t;
     launch(c, MHD_K_PREV, dim3(1), dim3(64), k_set2, c->scal + SC_TAU, t, t);
     do_fill(c);
-    do_bxyz(c);
+    end_of_stage(c);
     double out = t;
     if (c->p.stepop == 1) {
         do_dt(c, true);
         // hard-coded Adams-Bashforth start (mhd3d3.f90:3358-3373)
         out = 3.e-4;
         launch(c, MHD_K_PREV, dim3(1), dim3(64), k_set2, c->scal + SC_TAU, out, out);
-        double* x[11] = {c->rho, c->vx, c->vy, c->vz, c->pr, c->bx, c->by, c->bz, c->bi, c->bj, c->bk};
         for (int a = 0; a < 11; ++a) {
             const long n = a < 8 ? c->ncell : (a == 8 ? c->nbi : (a == 9 ? c->nbj : c->nbk));
-            if (hipMemcpyAsync(c->xp[a], x[a], n * sizeof(double), hipMemcpyDeviceToDevice, c->s) != hipSuccess)
+            if (hipMemcpyAsync(c->xp[a], *state_slot(c, a), n * sizeof(double), hipMemcpyDeviceToDevice, c->s) !=
+                hipSuccess)
                 return MHD_ERR_HIP;
         }
     }
@@ -384,43 +437,66 @@ int mhd_initialize(mhd_ctx* c, double* tau) {
     return sync_status(c);
 }
 
-// step_rk2 / step_rk3 (mhd3d3.f90:3648-3792)
+// step_rk2 / step_rk3 (mhd3d3.f90:3648-3792), in place
 template <int NST>
 static void step_rk(mhd_ctx* c) {
     do_dt(c, false);
     launch(c, MHD_K_PREV, dim3(1), dim3(64), k_set2, c->scal + SC_TAU, c->tau0_host, c->tau0_host);
     do_prev_rk(c);
+    double* self[11];
+    for (int a = 0; a < 11; ++a) self[a] = *state_slot(c, a);
     for (int st = 1; st <= NST; ++st) {
-        do_ops(c, cur_state(c), c->bi, c->bj, c->bk, c->vf);
-        if (st == 1) do_update<NST, 1>(c);
-        else if (st == 2) do_update<NST, 2>(c);
-        else do_update<NST, 3>(c);
+        if (mixed(c)) do_ops32(c);
+        else do_ops64(c, cur_src(c));
+        if (st == 1) do_update<NST, 1>(c, self);
+        else if (st == 2) do_update<NST, 2>(c, self);
+        else do_update<NST, 3>(c, self);
         do_fill(c);
-        do_bxyz(c);
+        end_of_stage(c);
     }
 }
 
-// step_ab (mhd3d3.f90:3602-3644)
+// step_ab (mhd3d3.f90:3602-3644): the new state goes to the next-state
+// buffers, then (previous, current, next) rotate -- the reference's
+// advance_previous copy x_p = x without moving data
 static void step_ab(mhd_ctx* c) {
     do_dt(c, true);
-    AB11 s;
-    double* x[11] = {c->rho, c->vx, c->vy, c->vz, c->pr, c->bx, c->by, c->bz, c->bi, c->bj, c->bk};
-    for (int a = 0; a < 11; ++a) {
-        s.x[a] = x[a];
-        s.xp[a] = c->xp[a];
-        s.xh[a] = c->xh[a];
+    const long nc = c->ncell, nbi = c->nbi, nbj = c->nbj, nbk = c->nbk;
+    if (mixed(c)) {
+        AB11<float> s;
+        for (int a = 0; a < 11; ++a) {
+            s.x[a] = *state_slot(c, a);
+            s.xp[a] = c->xp[a];
+        }
+        // extrapolated state straight into the fp32 stencil copies (sh order: .. bz vf bi bj bk)
+        for (int a = 0; a < 8; ++a) s.xh[a] = c->sh[a];
+        for (int a = 8; a < 11; ++a) s.xh[a] = c->sh[a + 1];
+        launch(c, MHD_K_PREV, blocks(c->nmax), dim3(kBlock), k_predict_ab<float>, s,
+               (const double*)(c->scal + SC_TAU), c->sh[8], nc, nbi, nbj, nbk);
+        launch(c, MHD_K_PREV, dim3(1), dim3(64), k_tau0_from_tau, c->scal + SC_TAU);
+        do_ops32(c);
+    } else {
+        AB11<double> s;
+        for (int a = 0; a < 11; ++a) {
+            s.x[a] = *state_slot(c, a);
+            s.xp[a] = c->xp[a];
+            s.xh[a] = c->xh[a];
+        }
+        launch(c, MHD_K_PREV, blocks(c->nmax), dim3(kBlock), k_predict_ab<double>, s,
+               (const double*)(c->scal + SC_TAU), c->vfh, nc, nbi, nbj, nbk);
+        launch(c, MHD_K_PREV, dim3(1), dim3(64), k_tau0_from_tau, c->scal + SC_TAU);
+        do_ops64(c, h_src(c));
     }
-    long n = c->ncell;
-    n = c->nbi > n ? c->nbi : n;
-    n = c->nbj > n ? c->nbj : n;
-    n = c->nbk > n ? c->nbk : n;
-    launch(c, MHD_K_PREV, blocks(n), dim3(kBlock), k_predict_ab, s, (const double*)(c->scal + SC_TAU), c->vfh,
-           c->ncell, c->nbi, c->nbj, c->nbk);
-    launch(c, MHD_K_PREV, dim3(1), dim3(64), k_tau0_from_tau, c->scal + SC_TAU);
-    do_ops(c, h_state(c), c->xh[8], c->xh[9], c->xh[10], c->vfh);
-    do_update<0, 0>(c);
+    do_update<0, 0>(c, c->xn);
+    // rotate: previous <- current <- next <- previous
+    for (int a = 0; a < 11; ++a) {
+        double* cur = *state_slot(c, a);
+        *state_slot(c, a) = c->xn[a];
+        c->xn[a] = c->xp[a];
+        c->xp[a] = cur;
+    }
     do_fill(c);
-    do_bxyz(c);
+    end_of_stage(c);
 }
 
 int mhd_step(mhd_ctx* c, int nsteps, double* tau, double* tau_cfl) {
@@ -432,8 +508,8 @@ int mhd_step(mhd_ctx* c, int nsteps, double* tau, double* tau_cfl) {
         else if (c->p.stepop == 2) step_rk<2>(c);
         else step_rk<3>(c);
     }
-    double h[2] = {0., 0.};
-    if (hipMemcpyAsync(h, c->scal, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s) != hipSuccess)
+    double h[3] = {0., 0., 0.};
+    if (hipMemcpyAsync(h, c->scal, 3 * sizeof(double), hipMemcpyDeviceToHost, c->s) != hipSuccess)
         return MHD_ERR_HIP;
     int rc = sync_status(c);
     if (tau) *tau = h[SC_TAU];

@@ -48,9 +48,8 @@ struct Prim {  // cell-centred state read by the flux and EMF kernels
 // face flux and magnetic stress: get_flux_stress (mhd3d3.f90:2385-2560)
 // F holds 8 arrays of nface each: rho, rvx, rvy, rvz, eng, sx, sy, sz
 // ------------------------------------------------------------------------
-template <typename R, int M, int D, typename TF>
-__global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<double> q, TF* __restrict__ F,
-                                                long nface) {
+template <typename R, int M, int D, typename TI, typename TF>
+__global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<TI> q, TF* __restrict__ F, long nface) {
     const long t = gtid();
     if (t >= nface) return;
     const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
@@ -115,137 +114,6 @@ __global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<double>
     F[5 * nface + t] = TF(sx + tx);
     F[6 * nface + t] = TF(sy + ty);
     F[7 * nface + t] = TF(sz + tz);
-}
-
-// ------------------------------------------------------------------------
-// corner (edge) velocity: get_vav_center2corner (mhd3d3.f90:1858-2116)
-// b = cell index of (I,J,K), the cell above the edge in both transverse
-// axes; s1/s2 = strides along the vav axes d1/d2 of the edge direction.
-// ------------------------------------------------------------------------
-template <typename R, int M>
-__device__ __forceinline__ R vav(const double* __restrict__ lv, ptrdiff_t b, ptrdiff_t s1, ptrdiff_t s2,
-                                 const Coef<R>& c) {
-    auto cv = [&](int o1, int o2) -> R { return R(lv[b + o1 * s1 + o2 * s2]); };
-    R v00_0, v10_0, v01_0, v11_0, v00_1, v01_1, v10_1, v11_1;
-    if constexpr (M == PDM) {
-        // interp2_pdm: along d1 at 8 levels of d2, then along d2 (926-989)
-        R g[8];
-#pragma unroll
-        for (int m2 = 0; m2 < 8; ++m2) {
-            R s[8];
-#pragma unroll
-            for (int m1 = 0; m1 < 8; ++m1) s[m1] = cv(m1 - 4, m2 - 4);
-            g[m2] = interp8(s);
-        }
-        const R vi = interp8(g);
-        // one-sided PDM2 estimates (1899-1912)
-        pdm2(cv(-2, -1), cv(-1, -1), cv(0, -1), cv(1, -1), vi, c.pdmb, v00_0, v10_0);
-        pdm2(cv(-1, -2), cv(-1, -1), cv(-1, 0), cv(-1, 1), vi, c.pdmb, v00_1, v01_1);
-        pdm2(cv(0, -2), cv(0, -1), cv(0, 0), cv(0, 1), vi, c.pdmb, v10_1, v11_1);
-        pdm2(cv(-2, 0), cv(-1, 0), cv(0, 0), cv(1, 0), vi, c.pdmb, v01_0, v11_0);
-    } else if constexpr (M == TVD) {
-        // interp2_tvd (1037-1067); TVD ignores the shifts (1187-1238)
-        const R vi = (cv(-1, -1) + cv(0, -1) + cv(0, 0) + cv(-1, 0)) / R(4);
-        tvd(cv(-2, 0), cv(-1, 0), cv(0, 0), cv(1, 0), vi, c.eps, v00_0, v10_0);
-        tvd(cv(0, -2), cv(0, -1), cv(0, 0), cv(0, 1), vi, c.eps, v00_1, v01_1);
-        v10_1 = v00_1;
-        v11_1 = v01_1;
-        v01_0 = v00_0;
-        v11_0 = v10_0;
-    } else {
-        // two-sweep WENO5/MPWENO5 corner values (1936-2051): family 1 sweeps
-        // d1 then d2, family 0 d2 then d1; digit 1 = side of the first
-        // sweep, digit 2 = side of the second sweep, in both families.
-        using S = Stencil<R, M>;
-        R L[6], Q[6], s[6];
-#pragma unroll
-        for (int mb = 0; mb < 6; ++mb) {
-#pragma unroll
-            for (int ma = 0; ma < 6; ++ma) s[ma] = cv(ma - 3, mb - 3);
-            S::run(s, c, L[mb], Q[mb]);
-        }
-        S::run(L, c, v00_1, v01_1);
-        S::run(Q, c, v10_1, v11_1);
-#pragma unroll
-        for (int mb = 0; mb < 6; ++mb) {
-#pragma unroll
-            for (int ma = 0; ma < 6; ++ma) s[ma] = cv(mb - 3, ma - 3);
-            S::run(s, c, L[mb], Q[mb]);
-        }
-        S::run(L, c, v00_0, v01_0);
-        S::run(Q, c, v10_0, v11_0);
-    }
-    // pick per quadrant by the larger one-sided jump (2058-2106)
-    R d00 = sgn1(fabs(cv(-2, -1) - cv(-1, -1)) - fabs(cv(-1, -2) - cv(-1, -1)));
-    R d10 = sgn1(fabs(cv(1, -1) - cv(0, -1)) - fabs(cv(0, -2) - cv(0, -1)));
-    R d01 = sgn1(fabs(cv(-2, 0) - cv(-1, 0)) - fabs(cv(-1, 1) - cv(-1, 0)));
-    R d11 = sgn1(fabs(cv(1, 0) - cv(0, 0)) - fabs(cv(0, 1) - cv(0, 0)));
-    R v00 = (R(1) + d00) / R(2) * v00_0 + (R(1) - d00) / R(2) * v00_1;
-    R v10 = (R(1) + d10) / R(2) * v10_0 + (R(1) - d10) / R(2) * v10_1;
-    R v01 = (R(1) + d01) / R(2) * v01_0 + (R(1) - d01) / R(2) * v01_1;
-    R v11 = (R(1) + d11) / R(2) * v11_0 + (R(1) - d11) / R(2) * v11_1;
-    return R(.25) * (v00 + v01 + v10 + v11);
-}
-
-// ------------------------------------------------------------------------
-// edge EMF: get_eijk_LFM (mhd3d3.f90:1754-1856).  K = edge direction.
-//   K=1 ei(nx0,ny0+1,nz0+1): v1=vy v2=vz, b1=bj (recon z), b2=bk (recon y)
-//   K=2 ej(nx0+1,ny0,nz0+1): v1=vz v2=vx, b1=bk (recon x), b2=bi (recon z)
-//   K=3 ek(nx0+1,ny0+1,nz0): v1=vx v2=vy, b1=bi (recon y), b2=bj (recon x)
-// ------------------------------------------------------------------------
-template <typename R, int M, int K, typename TF>
-__global__ void __launch_bounds__(kBlock) k_emf(Dims d, Coef<R> c, const double* __restrict__ v1p,
-                                               const double* __restrict__ v2p, const double* __restrict__ b1p,
-                                               const double* __restrict__ b2p, const double* __restrict__ vf,
-                                               TF* __restrict__ e, long nedge) {
-    const long t = gtid();
-    if (t >= nedge) return;
-    const int ii = d.nx0 + (K != 1), jj = d.ny0 + (K != 2);
-    int i, j, k;
-    unflat(t, ii, jj, i, j, k);
-    const int I = d.ng + i, J = d.ng + j, Kc = d.ng + k;
-    const int nx = d.nx, ny = d.ny;
-    const ptrdiff_t sx = 1, sy = nx, sz = (ptrdiff_t)nx * ny;
-    const ptrdiff_t bc = ix(nx, ny, I, J, Kc);
-    const ptrdiff_t s1 = K == 1 ? sy : (K == 2 ? sz : sx);
-    const ptrdiff_t s2 = K == 1 ? sz : (K == 2 ? sx : sy);
-    const R v1av = vav<R, M>(v1p, bc, s1, s2, c);
-    const R v2av = vav<R, M>(v2p, bc, s1, s2, c);
-    const R sig1 = sgn1(v1av), sig2 = sgn1(v2av);
-
-    ptrdiff_t base1, st1, base2, st2;
-    if (K == 1) {  // b1 = bj (nx,ny+1,nz) along z; b2 = bk (nx,ny,nz+1) along y
-        base1 = ix(nx, ny + 1, I, J, Kc);
-        st1 = (ptrdiff_t)nx * (ny + 1);
-        base2 = ix(nx, ny, I, J, Kc);
-        st2 = nx;
-    } else if (K == 2) {  // b1 = bk along x; b2 = bi (nx+1,ny,nz) along z
-        base1 = ix(nx, ny, I, J, Kc);
-        st1 = 1;
-        base2 = ix(nx + 1, ny, I, J, Kc);
-        st2 = (ptrdiff_t)(nx + 1) * ny;
-    } else {  // b1 = bi along y; b2 = bj along x
-        base1 = ix(nx + 1, ny, I, J, Kc);
-        st1 = nx + 1;
-        base2 = ix(nx, ny + 1, I, J, Kc);
-        st2 = 1;
-    }
-    R b1l, b1r, b2l, b2r;
-    recon_line<R, M>(b1p, base1, st1, c, b1l, b1r);
-    recon_line<R, M>(b2p, base2, st2, c, b2l, b2r);
-    const R b1u = (R(1) + sig2) / R(2) * b1l + (R(1) - sig2) / R(2) * b1r;
-    const R b2u = (R(1) + sig1) / R(2) * b2l + (R(1) - sig1) / R(2) * b2r;
-    R dum = -v1av * b2u + v2av * b1u;
-    // Alfven resistivity: mean of |v|+|B|/sqrt(rho) over the 4 cells at the edge
-    R eta;
-    if (K == 1)
-        eta = R(.25) * (R(vf[bc - sy - sz]) + R(vf[bc - sz]) + R(vf[bc]) + R(vf[bc - sy]));
-    else if (K == 2)
-        eta = R(.25) * (R(vf[bc - sx - sz]) + R(vf[bc - sx]) + R(vf[bc]) + R(vf[bc - sz]));
-    else
-        eta = R(.25) * (R(vf[bc - sx - sy]) + R(vf[bc - sy]) + R(vf[bc]) + R(vf[bc - sx]));
-    dum = dum + eta * (b1l + b2r - b1r - b2l);
-    e[t] = TF(dum);
 }
 
 // ------------------------------------------------------------------------

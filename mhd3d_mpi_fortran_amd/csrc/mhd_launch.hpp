@@ -8,6 +8,14 @@
 #include "../../include/mhd_gpu.h"
 #include "mhd_kernels.hpp"
 
+namespace mhd {
+// stencil inputs of one state: cell primitives, cell B, vf, face B
+template <typename T>
+struct Src {
+    const T *rho, *vx, *vy, *vz, *p, *bx, *by, *bz, *vf, *bi, *bj, *bk;
+};
+}  // namespace mhd
+
 // device scalars: scal[SC_TAU] = tau, scal[SC_TAU0] = tau0, scal[SC_DTCFL] = get_dt result
 enum { SC_TAU = 0, SC_TAU0 = 1, SC_DTCFL = 2 };
 
@@ -20,7 +28,7 @@ struct mhd_ctx {
     char* arena = nullptr;
     size_t arena_bytes = 0;
     bool have_geometry = false;
-    long ncell = 0, nbi = 0, nbj = 0, nbk = 0, ninterior = 0, ndtpart = 0;
+    long ncell = 0, nbi = 0, nbj = 0, nbk = 0, ninterior = 0, ndtpart = 0, nmax = 0;
     long nf[3] = {0, 0, 0}, ne[3] = {0, 0, 0};
     // state (reference layout)
     double *rho = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr, *pr = nullptr;
@@ -30,10 +38,16 @@ struct mhd_ctx {
     // RK advance_previous
     double *rhop = nullptr, *mxp = nullptr, *myp = nullptr, *mzp = nullptr, *engp = nullptr;
     double *bip = nullptr, *bjp = nullptr, *bkp = nullptr;
-    // AB previous / extrapolated (rho vx vy vz p bx by bz bi bj bk)
+    // AB: previous and next state (rotated with the current one each step),
+    // extrapolated state (binary64 mode; mixed mode extrapolates into sh)
+    // order: rho vx vy vz p bx by bz bi bj bk
     double* xp[11] = {};
+    double* xn[11] = {};
     double* xh[11] = {};
     double* vfh = nullptr;
+    // fp32 stencil inputs (mixed precision): rho vx vy vz p bx by bz vf bi bj bk
+    float* sh[12] = {};
+    bool dt_fresh = false;   // dtpart holds the CFL partials of the current state
     // per-stage products (binary64, or fp32 in mixed precision)
     void* flux[3] = {nullptr, nullptr, nullptr};
     void* emf[3] = {nullptr, nullptr, nullptr};
@@ -102,13 +116,11 @@ inline mhd::Coef<R> make_coef(const mhd_ctx* c) {
 }
 
 // per-precision stencil launchers (separate translation units)
-void launch_flux_fp64(mhd_ctx* c, mhd::Prim<double> q);
-void launch_emf_fp64(mhd_ctx* c, mhd::Prim<double> q, const double* bi, const double* bj, const double* bk,
-                     const double* vf);
+void launch_flux_fp64(mhd_ctx* c, mhd::Src<double> s);
+void launch_emf_fp64(mhd_ctx* c, mhd::Src<double> s);
 void launch_recon(mhd_ctx* c, int dir, const double* f, int fn1, int fn2, int lnx, int lny, int lnz, double* fl,
                   double* fr);
-void launch_flux_fp32(mhd_ctx* c, mhd::Prim<double> q);
-void launch_emf_fp32(mhd_ctx* c, mhd::Prim<double> q, const double* bi, const double* bj, const double* bk,
-                     const double* vf);
-void launch_flux_surrogate(mhd_ctx* c, mhd::Prim<double> q);
+void launch_flux_fp32(mhd_ctx* c, mhd::Src<float> s);
+void launch_emf_fp32(mhd_ctx* c, mhd::Src<float> s);
+void launch_flux_surrogate(mhd_ctx* c, mhd::Src<float> s);
 void surrogate_free(mhd_ctx* c);

@@ -5,9 +5,6 @@
 
 using namespace mhd;
 
-void launch_flux_fp32(mhd_ctx* c, Prim<double> q) { flux_launch<float, float>(c, q); }
+void launch_flux_fp32(mhd_ctx* c, Src<float> s) { flux_launch<float, float, float>(c, s); }
 
-void launch_emf_fp32(mhd_ctx* c, Prim<double> q, const double* bi, const double* bj, const double* bk,
-                     const double* vf) {
-    emf_launch<float, float>(c, q, bi, bj, bk, vf);
-}
+void launch_emf_fp32(mhd_ctx* c, Src<float> s) { emf_launch<float, float, float>(c, s); }

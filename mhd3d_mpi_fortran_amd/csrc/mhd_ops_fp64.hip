@@ -4,12 +4,9 @@
 
 using namespace mhd;
 
-void launch_flux_fp64(mhd_ctx* c, Prim<double> q) { flux_launch<double, double>(c, q); }
+void launch_flux_fp64(mhd_ctx* c, Src<double> s) { flux_launch<double, double, double>(c, s); }
 
-void launch_emf_fp64(mhd_ctx* c, Prim<double> q, const double* bi, const double* bj, const double* bk,
-                     const double* vf) {
-    emf_launch<double, double>(c, q, bi, bj, bk, vf);
-}
+void launch_emf_fp64(mhd_ctx* c, Src<double> s) { emf_launch<double, double, double>(c, s); }
 
 void launch_recon(mhd_ctx* c, int dir, const double* f, int fn1, int fn2, int lnx, int lny, int lnz, double* fl,
                   double* fr) {

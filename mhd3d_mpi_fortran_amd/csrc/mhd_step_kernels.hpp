@@ -14,8 +14,11 @@ namespace mhd {
 //   NST = 0: Adams-Bashforth corrector (no combine, p from the new momentum)
 //   NST = 2/3: TVD RK stage ST
 // ------------------------------------------------------------------------
-struct Cells {  // mutable cell state (in place)
+struct Cells {  // cell primitives written by the update
     double *rho, *vx, *vy, *vz, *p;
+};
+struct CCells {  // cell primitives the update starts from
+    const double *rho, *vx, *vy, *vz, *p;
 };
 struct PrevRK {  // advance_previous, RK flavour
     const double *rho, *mx, *my, *mz, *eng;
@@ -28,8 +31,8 @@ struct FluxSet {
 };
 
 template <int NST, int ST, typename TF>
-__global__ void __launch_bounds__(kBlock) k_update(Dims d, double gm1, const double* __restrict__ tau_p, Cells s,
-                                                  PrevRK pv, FluxSet<TF> F, const double* __restrict__ dx,
+__global__ void __launch_bounds__(kBlock) k_update(Dims d, double gm1, const double* __restrict__ tau_p, CCells s,
+                                                  Cells so, PrevRK pv, FluxSet<TF> F, const double* __restrict__ dx,
                                                   const double* __restrict__ dy, const double* __restrict__ dz,
                                                   long ncell) {
     const long t = gtid();
@@ -95,11 +98,11 @@ __global__ void __launch_bounds__(kBlock) k_update(Dims d, double gm1, const dou
         my = (pv.my[q] + 2. * my) / 3.;
         mz = (pv.mz[q] + 2. * mz) / 3.;
     }
-    s.rho[q] = rho;
-    s.p[q] = p;
-    s.vx[q] = mx / rho;
-    s.vy[q] = my / rho;
-    s.vz[q] = mz / rho;
+    so.rho[q] = rho;
+    so.p[q] = p;
+    so.vx[q] = mx / rho;
+    so.vy[q] = my / rho;
+    so.vz[q] = mz / rho;
 }
 
 // ------------------------------------------------------------------------
@@ -114,7 +117,7 @@ struct CFaces {
 };
 
 template <int NST, int ST, typename TF>
-__global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict__ tau_p, Faces b, CFaces bp,
+__global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict__ tau_p, CFaces b, Faces bo, CFaces bp,
                                               const TF* __restrict__ ei, const TF* __restrict__ ej,
                                               const TF* __restrict__ ek, const double* __restrict__ dx,
                                               const double* __restrict__ dy, const double* __restrict__ dz) {
@@ -138,7 +141,7 @@ __global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict_
         q = ix(d.nx + 1, d.ny, ng + i, ng + j, ng + k);
         old = b.bi[q];
         val = old - tau * (dek / dy[ng + j - 1] - dej / dz[ng + k - 1]);
-        out = b.bi;
+        out = bo.bi;
         prev = bp.bi;
     } else if (t < nbi + nbj) {
         unflat(t - nbi, nx0, ny0 + 1, i, j, k);
@@ -149,7 +152,7 @@ __global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict_
         q = ix(d.nx, d.ny + 1, ng + i, ng + j, ng + k);
         old = b.bj[q];
         val = old - tau * (dei / dz[ng + k - 1] - dek / dx[ng + i - 1]);
-        out = b.bj;
+        out = bo.bj;
         prev = bp.bj;
     } else {
         unflat(t - nbi - nbj, nx0, ny0, i, j, k);
@@ -160,7 +163,7 @@ __global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict_
         q = ix(d.nx, d.ny, ng + i, ng + j, ng + k);
         old = b.bk[q];
         val = old - tau * (dej / dx[ng + i - 1] - dei / dy[ng + j - 1]);
-        out = b.bk;
+        out = bo.bk;
         prev = bp.bk;
     }
     (void)old;
@@ -238,45 +241,6 @@ __global__ void __launch_bounds__(kBlock) k_fill(FillSet fs, int ng) {
     A.a[ix(len[0], len[1], i, j, k)] = A.a[ix(len[0], len[1], si, sj, sk)];
 }
 
-// ------------------------------------------------------------------------
-// get_bxyz (mhd3d3.f90:2118-2128) over the ghost-inclusive block, plus the
-// EMF's fast speed vf=|v|+|B|/sqrt(rho) (mhd3d3.f90:1766) of the same state
-// ------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_bxyz(Dims d, const double* __restrict__ bi, const double* __restrict__ bj,
-                                                const double* __restrict__ bk, const double* __restrict__ rho,
-                                                const double* __restrict__ vx, const double* __restrict__ vy,
-                                                const double* __restrict__ vz, double* __restrict__ bx,
-                                                double* __restrict__ by, double* __restrict__ bz,
-                                                double* __restrict__ vf, long ncell) {
-    const long t = gtid();
-    if (t >= ncell) return;
-    int i, j, k;
-    unflat(t, d.nx, d.ny, i, j, k);
-    const double x = .5 * (bi[ix(d.nx + 1, d.ny, i + 1, j, k)] + bi[ix(d.nx + 1, d.ny, i, j, k)]);
-    const double y = .5 * (bj[ix(d.nx, d.ny + 1, i, j + 1, k)] + bj[ix(d.nx, d.ny + 1, i, j, k)]);
-    const double z = .5 * (bk[t + (ptrdiff_t)d.nx * d.ny] + bk[t]);
-    bx[t] = x;
-    by[t] = y;
-    bz[t] = z;
-    if (vf) {
-        const double u = vx[t], v = vy[t], w = vz[t];
-        vf[t] = sqrt(u * u + v * v + w * w) + sqrt((x * x + y * y + z * z) / rho[t]);
-    }
-}
-
-// vf of an arbitrary state (get_eijk_LFM on a given state, mhd3d3.f90:1766)
-__global__ void __launch_bounds__(kBlock) k_vf(Prim<double> q, double* __restrict__ vf, long ncell) {
-    const long t = gtid();
-    if (t >= ncell) return;
-    const double u = q.vx[t], v = q.vy[t], w = q.vz[t], x = q.bx[t], y = q.by[t], z = q.bz[t];
-    vf[t] = sqrt(u * u + v * v + w * w) + sqrt((x * x + y * y + z * z) / q.rho[t]);
-}
-
-// ------------------------------------------------------------------------
-// CFL timestep: get_dt (mhd3d3.f90:1608-1639), min over the ghost-inclusive
-// block.  Pass 1: per-workgroup min (wavefront shuffles + LDS); pass 2: one
-// workgroup reduces the partials.  Deterministic, no atomics, no memset.
-// ------------------------------------------------------------------------
 __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -301,6 +265,87 @@ __device__ __forceinline__ double block_min(double v) {
     return red[0];
 }
 
+// ------------------------------------------------------------------------
+// get_bxyz (mhd3d3.f90:2118-2128) over the ghost-inclusive block, fused with
+//   * the EMF's fast speed vf=|v|+|B|/sqrt(rho) (mhd3d3.f90:1766) when vf != 0,
+//   * fp32 copies of the stencil inputs (mixed precision) when sh.rho != 0,
+//   * per-workgroup minima of the CFL dt (get_dt, mhd3d3.f90:1608-1628)
+//     when part != 0 -- the dt the next step starts with.
+// Threads cover max(ncell, nbi, nbj, nbk) so the face copies ride along.
+// ------------------------------------------------------------------------
+struct Shadow {  // fp32 stencil inputs of one state
+    float *rho, *vx, *vy, *vz, *p, *bx, *by, *bz, *vf, *bi, *bj, *bk;
+};
+
+__device__ __forceinline__ double cfl_dt(double gam, double cfl, double u, double v, double w, double x, double y,
+                                         double z, double r, double pp, double dxi, double dyj, double dzk) {
+    const double vv = sqrt(u * u + v * v + w * w);
+    const double btot = sqrt(x * x + y * y + z * z);
+    const double valf = btot / sqrt(r);
+    const double vs = sqrt(gam * pp / r);
+    const double vcfl = vv + sqrt(valf * valf + vs * vs);
+    return cfl / (vcfl / dxi + vcfl / dyj + vcfl / dzk);
+}
+
+__global__ void __launch_bounds__(kBlock) k_bxyz(Dims d, double gam, double cfl, CFaces b, CCells s,
+                                                double* __restrict__ bx, double* __restrict__ by,
+                                                double* __restrict__ bz, double* __restrict__ vf, Shadow sh,
+                                                const double* __restrict__ dx, const double* __restrict__ dy,
+                                                const double* __restrict__ dz, double* __restrict__ part, long ncell,
+                                                long nbi, long nbj, long nbk) {
+    const long t = gtid();
+    double dt = __builtin_huge_val();
+    if (t < ncell) {
+        int i, j, k;
+        unflat(t, d.nx, d.ny, i, j, k);
+        const double x = .5 * (b.bi[ix(d.nx + 1, d.ny, i + 1, j, k)] + b.bi[ix(d.nx + 1, d.ny, i, j, k)]);
+        const double y = .5 * (b.bj[ix(d.nx, d.ny + 1, i, j + 1, k)] + b.bj[ix(d.nx, d.ny + 1, i, j, k)]);
+        const double z = .5 * (b.bk[t + (ptrdiff_t)d.nx * d.ny] + b.bk[t]);
+        bx[t] = x;
+        by[t] = y;
+        bz[t] = z;
+        const double u = s.vx[t], v = s.vy[t], w = s.vz[t], r = s.rho[t], pp = s.p[t];
+        if (vf || sh.rho) {
+            const double f = sqrt(u * u + v * v + w * w) + sqrt((x * x + y * y + z * z) / r);
+            if (vf) vf[t] = f;
+            if (sh.rho) {
+                sh.rho[t] = float(r);
+                sh.vx[t] = float(u);
+                sh.vy[t] = float(v);
+                sh.vz[t] = float(w);
+                sh.p[t] = float(pp);
+                sh.bx[t] = float(x);
+                sh.by[t] = float(y);
+                sh.bz[t] = float(z);
+                sh.vf[t] = float(f);
+            }
+        }
+        if (part) dt = cfl_dt(gam, cfl, u, v, w, x, y, z, r, pp, dx[i - 1], dy[j - 1], dz[k - 1]);
+    }
+    if (sh.bi) {
+        if (t < nbi) sh.bi[t] = float(b.bi[t]);
+        if (t < nbj) sh.bj[t] = float(b.bj[t]);
+        if (t < nbk) sh.bk[t] = float(b.bk[t]);
+    }
+    if (part) {
+        dt = block_min(dt);
+        if (threadIdx.x == 0) part[blockIdx.x] = dt;
+    }
+}
+
+// vf of an arbitrary state (get_eijk_LFM on a given state, mhd3d3.f90:1766)
+__global__ void __launch_bounds__(kBlock) k_vf(Prim<double> q, double* __restrict__ vf, long ncell) {
+    const long t = gtid();
+    if (t >= ncell) return;
+    const double u = q.vx[t], v = q.vy[t], w = q.vz[t], x = q.bx[t], y = q.by[t], z = q.bz[t];
+    vf[t] = sqrt(u * u + v * v + w * w) + sqrt((x * x + y * y + z * z) / q.rho[t]);
+}
+
+// ------------------------------------------------------------------------
+// CFL timestep: get_dt (mhd3d3.f90:1608-1639), min over the ghost-inclusive
+// block.  Pass 1: per-workgroup min (wavefront shuffles + LDS); pass 2: one
+// workgroup reduces the partials.  Deterministic, no atomics, no memset.
+// ------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_dt_partial(Dims d, double gam, double cfl, Prim<double> q,
                                                       const double* __restrict__ dx, const double* __restrict__ dy,
                                                       const double* __restrict__ dz, double* __restrict__ part,
@@ -310,14 +355,8 @@ __global__ void __launch_bounds__(kBlock) k_dt_partial(Dims d, double gam, doubl
     if (t < ncell) {
         int i, j, k;
         unflat(t, d.nx, d.ny, i, j, k);
-        const double u = q.vx[t], v = q.vy[t], w = q.vz[t], x = q.bx[t], y = q.by[t], z = q.bz[t];
-        const double r = q.rho[t];
-        const double vv = sqrt(u * u + v * v + w * w);
-        const double btot = sqrt(x * x + y * y + z * z);
-        const double valf = btot / sqrt(r);
-        const double vs = sqrt(gam * q.p[t] / r);
-        const double vcfl = vv + sqrt(valf * valf + vs * vs);
-        dt = cfl / (vcfl / dx[i - 1] + vcfl / dy[j - 1] + vcfl / dz[k - 1]);
+        dt = cfl_dt(gam, cfl, q.vx[t], q.vy[t], q.vz[t], q.bx[t], q.by[t], q.bz[t], q.rho[t], q.p[t], dx[i - 1],
+                    dy[j - 1], dz[k - 1]);
     }
     dt = block_min(dt);
     if (threadIdx.x == 0) part[blockIdx.x] = dt;
@@ -379,39 +418,40 @@ __global__ void __launch_bounds__(kBlock) k_prev_rk(double gm1, Prim<double> q, 
 }
 
 // ------------------------------------------------------------------------
-// Adams-Bashforth predictor (mhd3d3.f90:3821-3840) fused with
-// advance_previous (AB flavour) and the h-state's vf:
-//   xh = x + tau/tau0/2*(x - xp);  xp = x      for the 11 state arrays
+// Adams-Bashforth predictor (mhd3d3.f90:3821-3840):
+//   xh = x + tau/tau0/2*(x - xp)   for the 11 state arrays, plus the h-state vf
+// computed in binary64 and stored as TH (fp32 in mixed precision).  The
+// reference's advance_previous copy xp = x is a buffer rotation on the host.
 // ------------------------------------------------------------------------
+template <typename TH>
 struct AB11 {
-    double* x[11];   // rho vx vy vz p bx by bz bi bj bk  (read)
-    double* xp[11];  // previous (read, then overwritten by x)
-    double* xh[11];  // extrapolated (written)
+    const double* x[11];   // rho vx vy vz p bx by bz bi bj bk
+    const double* xp[11];
+    TH* xh[11];
 };
 
-__global__ void __launch_bounds__(kBlock) k_predict_ab(AB11 s, const double* __restrict__ tau_p,
-                                                      double* __restrict__ vfh, long ncell, long nbi, long nbj,
+template <typename TH>
+__global__ void __launch_bounds__(kBlock) k_predict_ab(AB11<TH> s, const double* __restrict__ tau_p,
+                                                      TH* __restrict__ vfh, long ncell, long nbi, long nbj,
                                                       long nbk) {
     const long t = gtid();
     const double c = tau_p[0] / tau_p[1] / 2.;
     if (t < ncell) {
+        double h[8];
 #pragma unroll
         for (int a = 0; a < 8; ++a) {
             const double x = s.x[a][t], xp = s.xp[a][t];
-            s.xh[a][t] = x + c * (x - xp);
-            s.xp[a][t] = x;
+            h[a] = x + c * (x - xp);
+            s.xh[a][t] = TH(h[a]);
         }
-        const double u = s.xh[1][t], v = s.xh[2][t], w = s.xh[3][t];
-        const double bx = s.xh[5][t], by = s.xh[6][t], bz = s.xh[7][t];
-        vfh[t] = sqrt(u * u + v * v + w * w) + sqrt((bx * bx + by * by + bz * bz) / s.xh[0][t]);
+        vfh[t] = TH(sqrt(h[1] * h[1] + h[2] * h[2] + h[3] * h[3]) + sqrt((h[5] * h[5] + h[6] * h[6] + h[7] * h[7]) / h[0]));
     }
     const long nf[3] = {nbi, nbj, nbk};
 #pragma unroll
     for (int a = 0; a < 3; ++a)
         if (t < nf[a]) {
             const double x = s.x[8 + a][t], xp = s.xp[8 + a][t];
-            s.xh[8 + a][t] = x + c * (x - xp);
-            s.xp[8 + a][t] = x;
+            s.xh[8 + a][t] = TH(x + c * (x - xp));
         }
 }
 

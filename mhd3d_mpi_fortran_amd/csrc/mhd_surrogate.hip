@@ -4,6 +4,6 @@
 
 using namespace mhd;
 
-void launch_flux_surrogate(mhd_ctx* c, Prim<double> q) { launch_flux_fp32(c, q); }
+void launch_flux_surrogate(mhd_ctx* c, Src<float> s) { launch_flux_fp32(c, s); }
 
 void surrogate_free(mhd_ctx* c) { (void)c; }
