@@ -127,6 +127,41 @@ enum { MHD_K_FLUX = 0, MHD_K_EMF = 1, MHD_K_UPDATE = 2, MHD_K_CT = 3, MHD_K_FILL
 int mhd_profile(mhd_ctx *ctx, int enable);
 int mhd_profile_read(mhd_ctx *ctx, int kclass, int *launches, double *ms);
 
+/* ---- learned face reconstruction (no counterpart in the reference) ------
+ * A net y = lin(x) + gate(x) * MLP(x), MLP 40 -> 32 -> 32 -> 16 (activation
+ * z*sigmoid(1.702 z)); per variable lin is the fixed 4th-order interpolation
+ * (-x0 + 7x1 + 7x2 - x3)/12 and gate the largest |second difference| of its
+ * window.  It maps the normalised 5-point
+ * stencils (cells j-2..j+2 of the face between j-1 and j) of the 8
+ * primitives to the face's left/right states; it replaces PDM in the fp32
+ * flux of a MHD_MIXED context created with surrogate = 1, with PDM as
+ * fallback where the largest normalised second difference exceeds theta or
+ * a density/pressure state is not positive.  Windows are centred on the
+ * mean of the two cells next to the face and divided by max(range, 1e-3 x
+ * the variable's physical scale); states are clamped to the two cells'
+ * range.
+ * Parameters: MHD_SUR_NPARAM fp32, packed W1[32][40] b1[32] W2[32][32]
+ * b2[32] W3[16][32] b3[16]; inference uses bf16 copies (MFMA), fp32
+ * accumulate.  Samples: 56 floats = 40 normalised inputs + 16 PDM targets. */
+enum { MHD_SUR_NPARAM = 2896, MHD_SUR_SAMPLE = 56 };
+int mhd_surrogate_set_weights(mhd_ctx *ctx, const float *w, int n);
+int mhd_surrogate_get_weights(mhd_ctx *ctx, float *w, int n);
+int mhd_surrogate_configure(mhd_ctx *ctx, double theta, int count_fallbacks);
+/* faces evaluated / PDM fallbacks since the last call (count_fallbacks on) */
+int mhd_surrogate_stats(mhd_ctx *ctx, long *faces, long *fallbacks);
+/* append face samples of the current state (faces with t % stride == phase,
+ * phase cycling per call) whose roughness <= theta; capacity = set size */
+int mhd_surrogate_collect(mhd_ctx *ctx, double theta, int stride, long capacity, long *collected);
+int mhd_surrogate_load_samples(mhd_ctx *ctx, const float *samples, long n);
+int mhd_surrogate_fetch_samples(mhd_ctx *ctx, float *out, long n);
+/* Adam training on the collected set (GPU forward/backward kernels) */
+int mhd_surrogate_train(mhd_ctx *ctx, int epochs, int batch, double lr, unsigned seed, double *mean_loss);
+/* loss and gradient of the given weights on n host samples (gradient checks);
+ * the context's own net and optimiser state are not touched */
+int mhd_surrogate_grad(mhd_ctx *ctx, const float *w, const float *samples, int n, double *loss, float *grad);
+/* bf16-MFMA forward of n host rows of 40 normalised inputs -> n rows of 16 */
+int mhd_surrogate_infer(mhd_ctx *ctx, const float *x, long n, float *y);
+
 #ifdef __cplusplus
 }
 #endif

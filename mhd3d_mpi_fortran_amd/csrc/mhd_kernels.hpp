@@ -48,25 +48,14 @@ struct Prim {  // cell-centred state read by the flux and EMF kernels
 // face flux and magnetic stress: get_flux_stress (mhd3d3.f90:2385-2560)
 // F holds 8 arrays of nface each: rho, rvx, rvy, rvz, eng, sx, sy, sz
 // ------------------------------------------------------------------------
-template <typename R, int M, int D, typename TI, typename TF>
-__global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<TI> q, TF* __restrict__ F, long nface) {
-    const long t = gtid();
-    if (t >= nface) return;
-    const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
-    int i, j, k;
-    unflat(t, ii, jj, i, j, k);
-    const ptrdiff_t base = ix(d.nx, d.ny, d.ng + i, d.ng + j, d.ng + k);
-    const ptrdiff_t st = D == 1 ? 1 : (D == 2 ? (ptrdiff_t)d.nx : (ptrdiff_t)d.nx * d.ny);
-    R rl, rr, ul, ur, vl, vr, wl, wr, pl, pr, bxl, bxr, byl, byr, bzl, bzr;
-    recon_line<R, M>(q.rho, base, st, c, rl, rr);
-    recon_line<R, M>(q.vx, base, st, c, ul, ur);
-    recon_line<R, M>(q.vy, base, st, c, vl, vr);
-    recon_line<R, M>(q.vz, base, st, c, wl, wr);
-    recon_line<R, M>(q.p, base, st, c, pl, pr);
-    recon_line<R, M>(q.bx, base, st, c, bxl, bxr);
-    recon_line<R, M>(q.by, base, st, c, byl, byr);
-    recon_line<R, M>(q.bz, base, st, c, bzl, bzr);
 
+// Gaussian-split flux of one face from its left/right states
+// l, r = (rho, vx, vy, vz, p, bx, by, bz), written at index t of F
+template <typename R, int D, typename TF>
+__device__ __forceinline__ void face_flux(const Coef<R>& c, const R* l, const R* r, TF* __restrict__ F, long t,
+                                          long nface) {
+    const R rl = l[0], ul = l[1], vl = l[2], wl = l[3], pl = l[4], bxl = l[5], byl = l[6], bzl = l[7];
+    const R rr = r[0], ur = r[1], vr = r[2], wr = r[3], pr = r[4], bxr = r[5], byr = r[6], bzr = r[7];
     R v0, v1;
     // left state, positive half-moments
     const R unl = D == 1 ? ul : (D == 2 ? vl : wl);
@@ -114,6 +103,27 @@ __global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<TI> q, 
     F[5 * nface + t] = TF(sx + tx);
     F[6 * nface + t] = TF(sy + ty);
     F[7 * nface + t] = TF(sz + tz);
+}
+
+template <typename R, int M, int D, typename TI, typename TF>
+__global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<TI> q, TF* __restrict__ F, long nface) {
+    const long t = gtid();
+    if (t >= nface) return;
+    const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
+    int i, j, k;
+    unflat(t, ii, jj, i, j, k);
+    const ptrdiff_t base = ix(d.nx, d.ny, d.ng + i, d.ng + j, d.ng + k);
+    const ptrdiff_t st = D == 1 ? 1 : (D == 2 ? (ptrdiff_t)d.nx : (ptrdiff_t)d.nx * d.ny);
+    R l[8], r[8];
+    recon_line<R, M>(q.rho, base, st, c, l[0], r[0]);
+    recon_line<R, M>(q.vx, base, st, c, l[1], r[1]);
+    recon_line<R, M>(q.vy, base, st, c, l[2], r[2]);
+    recon_line<R, M>(q.vz, base, st, c, l[3], r[3]);
+    recon_line<R, M>(q.p, base, st, c, l[4], r[4]);
+    recon_line<R, M>(q.bx, base, st, c, l[5], r[5]);
+    recon_line<R, M>(q.by, base, st, c, l[6], r[6]);
+    recon_line<R, M>(q.bz, base, st, c, l[7], r[7]);
+    face_flux<R, D, TF>(c, l, r, F, t, nface);
 }
 
 // ------------------------------------------------------------------------

@@ -67,6 +67,8 @@ struct mhd_ctx {
     double prof_ms[MHD_K_NCLASS] = {};
     int prof_n[MHD_K_NCLASS] = {};
     SurrogateState* sur = nullptr;
+    float sur_theta = 1.0f;    // smoothness threshold of the learned reconstruction
+    bool sur_stats = false;    // count faces / fallbacks
 };
 
 inline dim3 blocks(long n) { return dim3((unsigned)((n + mhd::kBlock - 1) / mhd::kBlock)); }

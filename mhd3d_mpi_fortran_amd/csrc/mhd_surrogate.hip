@@ -1,9 +1,805 @@
-// learned face reconstruction (placeholder until the surrogate kernels land):
-// the flux falls back to the fp32 PDM path.
+// Learned face reconstruction (no counterpart in the reference): a small MLP
+// maps the normalised 5-point stencils of the 8 primitives at a face to the
+// face's left/right states; plain PDM (mhd3d3.f90:862-924, 1069-1172) is the
+// fallback wherever a smoothness or positivity check fails, and it is the
+// judge the net is trained against.
+//
+//   input   x[5q+k] = (f_q(cell j-2+k) - c_q)/s_q, k=0..4, face between j-1, j
+//           c_q = (f_q(j-1)+f_q(j))/2, s_q = max(max_k |f_q(j-2+k) - c_q|,
+//           1e-3 * physical scale of q: |rho|, |p|, |v|+sqrt(p/rho) or
+//           |B|+sqrt(p) from the centres), so variations far below a
+//           variable's scale stay small
+//   output  y[q] = (L_q - c_q)/s_q, y[8+q] = (R_q - c_q)/s_q; in the solver
+//           clamped to [min, max] of f(j-1), f(j), PDM's own bound
+//   net     y = lin(x) + gate(x) * MLP(x): MLP 40 -> 32 -> 32 -> 16,
+//           activation g(z) = z*sigmoid(1.702 z) (the sigmoid form of GELU),
+//           fp32 master weights; per output of variable q, lin = the fixed
+//           4th-order interpolation (-x0 + 7 x1 + 7 x2 - x3)/12 of q's window
+//           and gate = the largest |second difference| of q's window, so the
+//           MLP learns PDM's correction to lin and the correction vanishes
+//           on linear data
+//   parameters, packed: W1[32][40] b1[32] W2[32][32] b2[32] W3[16][32] b3[16]
+//
+// Kernels: sample collection from the solver's own state (binary64 PDM
+// targets), training (per-sample forward/backward, per-weight gradient
+// reduction, Adam) and bf16-MFMA inference fused into the fp32 face flux.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
 #include "mhd_ops.hpp"
 
 using namespace mhd;
 
-void launch_flux_surrogate(mhd_ctx* c, Src<float> s) { launch_flux_fp32(c, s); }
+namespace sur {
 
-void surrogate_free(mhd_ctx* c) { (void)c; }
+constexpr int NI = 40, NH = 32, NO = 16;
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NH * NI, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH,
+              OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NO * NH, NPARAM = OFF_B3 + NO;  // 2896
+constexpr int NS = NI + NO;  // floats per training sample: inputs, targets
+constexpr float GK = 1.702f;
+
+__device__ __forceinline__ float act(float z) { return z / (1.f + expf(-GK * z)); }
+__device__ __forceinline__ float act_grad(float z) {
+    const float s = 1.f / (1.f + expf(-GK * z));
+    return s + GK * z * s * (1.f - s);
+}
+__device__ __forceinline__ float act_fast(float z) { return z * __frcp_rn(1.f + __expf(-GK * z)); }
+
+// 5-point window of one variable in arithmetic type A: values, centre
+// c = (f(j-1)+f(j))/2 and range m = max |w - c|
+template <typename A, typename T>
+__device__ __forceinline__ void load5(const T* __restrict__ f, ptrdiff_t base, ptrdiff_t st, A* w, A& c, A& m) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) w[k] = A(f[base + (ptrdiff_t)(k - 2) * st]);
+    c = A(.5) * (w[1] + w[2]);
+    m = A(0);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) m = fmax(m, fabs(w[k] - c));
+}
+
+// physical scale of each variable at the face from the window centres
+// c = (rho, vx, vy, vz, p, bx, by, bz)
+template <typename A>
+__device__ __forceinline__ void var_scales(const A* c, A* sc) {
+    const A r = fabs(c[0]), p = fabs(c[4]);
+    const A v = fmax(fmax(fabs(c[1]), fabs(c[2])), fabs(c[3])) + sqrt(p / fmax(r, A(1e-30)));
+    const A b = fmax(fmax(fabs(c[5]), fabs(c[6])), fabs(c[7])) + sqrt(p);
+    sc[0] = r;
+    sc[1] = sc[2] = sc[3] = v;
+    sc[4] = p;
+    sc[5] = sc[6] = sc[7] = b;
+}
+constexpr float NEGLIGIBLE = 1e-3f;
+
+// normalisation of a window: s = max(range, 1e-3 * physical scale), so that
+// variations far below the variable's scale stay small after normalisation
+// (they carry no weight in the loss and no roughness)
+template <typename A>
+__device__ __forceinline__ A norm_scale(A m, A sc) {
+    return fmax(fmax(m, A(NEGLIGIBLE) * sc), A(1e-30));
+}
+
+// the fixed linear part of the net: 4th-order interpolation of each window
+__device__ __forceinline__ float lin4(const float* x) {
+    return (-x[0] + 7.f * x[1] + 7.f * x[2] - x[3]) * (1.f / 12.f);
+}
+
+// smoothness of a normalised window: largest normalised second difference;
+// it is also the gate of the learned correction of that variable's outputs
+__device__ __forceinline__ float rough5(const float* x) {
+    float r = 0.f;
+#pragma unroll
+    for (int k = 1; k < 4; ++k) r = fmaxf(r, fabsf(x[k - 1] - 2.f * x[k] + x[k + 1]));
+    return r;
+}
+
+struct State {
+    float* p = nullptr;        // fp32 master parameters
+    float* g = nullptr;        // gradient accumulator
+    float* m = nullptr;        // Adam moments
+    float* v = nullptr;
+    float* samples = nullptr;  // [cap][NS]
+    long cap = 0;
+    long* count = nullptr;     // device counter of collected samples
+    float* scratch = nullptr;  // per-sample activations/deltas of one minibatch
+    long scratch_n = 0;
+    float* loss = nullptr;     // device loss accumulator
+    unsigned short* wbf = nullptr;  // bf16 copies for inference: W1t[40][32] W2t[32][32] W3t[32][32]
+    float* bias = nullptr;     // b1[32] b2[32] b3[16] (fp32)
+    unsigned long long* stats = nullptr;  // faces evaluated, fallbacks
+    long step = 0;
+    bool have_weights = false;
+};
+
+// ---------------------------------------------------------------------------
+// sample collection: faces of the current (binary64) state along direction D
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_collect(Dims d, Coef<double> c, Prim<double> q, float theta, int stride,
+                                                   int phase, float* __restrict__ out, long cap,
+                                                   long* __restrict__ count, long nface) {
+    const long t = gtid();
+    if (t >= nface || (t % stride) != phase) return;
+    const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
+    int i, j, k;
+    unflat(t, ii, jj, i, j, k);
+    const ptrdiff_t base = ix(d.nx, d.ny, d.ng + i, d.ng + j, d.ng + k);
+    const ptrdiff_t st = D == 1 ? 1 : (D == 2 ? (ptrdiff_t)d.nx : (ptrdiff_t)d.nx * d.ny);
+    const double* f[8] = {q.rho, q.vx, q.vy, q.vz, q.p, q.bx, q.by, q.bz};
+    double wd[NI], cc[8], mm[8], sc[8];
+    float x[NI], y[NO], rough = 0.f;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) load5<double>(f[v], base, st, wd + 5 * v, cc[v], mm[v]);
+    var_scales<double>(cc, sc);
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        const double ss = norm_scale<double>(mm[v], sc[v]), r = 1. / ss;
+#pragma unroll
+        for (int a = 0; a < 5; ++a) x[5 * v + a] = float((wd[5 * v + a] - cc[v]) * r);
+        rough = fmaxf(rough, rough5(x + 5 * v));
+        double fl, fr;
+        recon_line<double, PDM>(f[v], base, st, c, fl, fr);
+        y[v] = float((fl - cc[v]) * r);
+        y[8 + v] = float((fr - cc[v]) * r);
+    }
+    if (!(rough <= theta)) return;
+    const long slot = atomicAdd((unsigned long long*)count, 1ull);
+    if (slot >= cap) return;
+    float* o = out + slot * NS;
+#pragma unroll
+    for (int a = 0; a < NI; ++a) o[a] = x[a];
+#pragma unroll
+    for (int a = 0; a < NO; ++a) o[NI + a] = y[a];
+}
+
+// ---------------------------------------------------------------------------
+// training
+// scratch per sample b: x(40) a1(32) a2(32) dz1(32) dz2(32) dy(16)
+// ---------------------------------------------------------------------------
+constexpr int SC = NI + 4 * NH + NO;  // 184
+
+// forward + backward of one minibatch; sample b of the batch is sample
+// (first + b*mult) mod n of the set (a permutation when gcd(mult, n) = 1)
+__global__ void __launch_bounds__(kBlock) k_train_fb(const float* __restrict__ P, const float* __restrict__ set,
+                                                    long n, long first, long mult, int batch, float inv_norm,
+                                                    float* __restrict__ scratch, float* __restrict__ loss) {
+    __shared__ float w[NPARAM];
+    for (int a = threadIdx.x; a < NPARAM; a += blockDim.x) w[a] = P[a];
+    __syncthreads();
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    float l2 = 0.f;
+    if (b < batch) {
+        const long idx = (long)(((unsigned long long)first + (unsigned long long)b * (unsigned long long)mult) % (unsigned long long)n);
+        const float* s = set + idx * NS;
+        float* sc = scratch + (long)b * SC;
+        float x[NI], z1[NH], a1[NH], z2[NH], a2[NH], y[NO];
+        for (int a = 0; a < NI; ++a) {
+            x[a] = s[a];
+            sc[a] = x[a];
+        }
+        for (int j = 0; j < NH; ++j) {
+            float acc = w[OFF_B1 + j];
+            for (int a = 0; a < NI; ++a) acc += w[OFF_W1 + j * NI + a] * x[a];
+            z1[j] = acc;
+            a1[j] = act(acc);
+            sc[NI + j] = a1[j];
+        }
+        for (int j = 0; j < NH; ++j) {
+            float acc = w[OFF_B2 + j];
+            for (int a = 0; a < NH; ++a) acc += w[OFF_W2 + j * NH + a] * a1[a];
+            z2[j] = acc;
+            a2[j] = act(acc);
+            sc[NI + NH + j] = a2[j];
+        }
+        float dy[NO];
+        for (int o = 0; o < NO; ++o) {
+            float acc = w[OFF_B3 + o];
+            for (int a = 0; a < NH; ++a) acc += w[OFF_W3 + o * NH + a] * a2[a];
+            const float gt = rough5(x + 5 * (o & 7));
+            y[o] = lin4(x + 5 * (o & 7)) + gt * acc;
+            const float e = y[o] - s[NI + o];
+            l2 += e * e;
+            dy[o] = 2.f * e * gt * inv_norm;
+            sc[NI + 4 * NH + o] = dy[o];
+        }
+        float dz2[NH];
+        for (int a = 0; a < NH; ++a) {
+            float acc = 0.f;
+            for (int o = 0; o < NO; ++o) acc += w[OFF_W3 + o * NH + a] * dy[o];
+            dz2[a] = acc * act_grad(z2[a]);
+            sc[NI + 3 * NH + a] = dz2[a];
+        }
+        for (int a = 0; a < NH; ++a) {
+            float acc = 0.f;
+            for (int j = 0; j < NH; ++j) acc += w[OFF_W2 + j * NH + a] * dz2[j];
+            sc[NI + 2 * NH + a] = acc * act_grad(z1[a]);
+        }
+    }
+    // block sum of the squared error
+    __shared__ float red[kBlock / 64];
+    for (int o = 32; o > 0; o >>= 1) l2 += __shfl_xor(l2, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = l2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float s = 0.f;
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += red[q];
+        atomicAdd(loss, s);
+    }
+}
+
+// dParam = sum over the batch of delta_out * activation_in; each workgroup
+// covers 64 parameters x 4 batch chunks, grid.y splits the batch further
+__global__ void __launch_bounds__(kBlock) k_train_grad(const float* __restrict__ scratch, int batch,
+                                                      float* __restrict__ g) {
+    const int pidx = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int nch = gridDim.y * 4;
+    const int ch = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (pidx >= NPARAM) return;
+    int din, ain;  // scratch offsets of delta and input activation (-1: bias)
+    if (pidx < OFF_B1) { din = NI + 2 * NH + pidx / NI; ain = pidx % NI; }
+    else if (pidx < OFF_W2) { din = NI + 2 * NH + (pidx - OFF_B1); ain = -1; }
+    else if (pidx < OFF_B2) { din = NI + 3 * NH + (pidx - OFF_W2) / NH; ain = NI + (pidx - OFF_W2) % NH; }
+    else if (pidx < OFF_W3) { din = NI + 3 * NH + (pidx - OFF_B2); ain = -1; }
+    else if (pidx < OFF_B3) { din = NI + 4 * NH + (pidx - OFF_W3) / NH; ain = NI + NH + (pidx - OFF_W3) % NH; }
+    else { din = NI + 4 * NH + (pidx - OFF_B3); ain = -1; }
+    const int per = (batch + nch - 1) / nch;
+    const int b0 = ch * per, b1 = b0 + per < batch ? b0 + per : batch;
+    float acc = 0.f;
+    for (int b = b0; b < b1; ++b) {
+        const float* sc = scratch + (long)b * SC;
+        acc += sc[din] * (ain >= 0 ? sc[ain] : 1.f);
+    }
+    atomicAdd(&g[pidx], acc);
+}
+
+__global__ void k_adam(float* __restrict__ P, float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       float lr, float b1c, float b2c) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= NPARAM) return;
+    const float gg = g[a];
+    const float mm = .9f * m[a] + .1f * gg;
+    const float vv = .999f * v[a] + .001f * gg * gg;
+    m[a] = mm;
+    v[a] = vv;
+    P[a] -= lr * (mm / b1c) / (sqrtf(vv / b2c) + 1e-8f);
+    g[a] = 0.f;
+}
+
+__global__ void k_zero_f(float* __restrict__ p, int n) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < n) p[a] = 0.f;
+}
+
+// bf16 (round to nearest even) copies of the weights, transposed to the
+// MFMA B-operand order [k][j]; W3 padded to 32 outputs with zeros
+__device__ __forceinline__ unsigned short to_bf16(float f) {
+    unsigned int u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__global__ void k_pack_bf16(const float* __restrict__ P, unsigned short* __restrict__ wbf, float* __restrict__ bias) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < NI * NH) {  // W1t[k][j]
+        const int k = a / NH, j = a % NH;
+        wbf[a] = to_bf16(P[OFF_W1 + j * NI + k]);
+    }
+    if (a < NH * NH) {  // W2t[k][j]
+        const int k = a / NH, j = a % NH;
+        wbf[NI * NH + a] = to_bf16(P[OFF_W2 + j * NH + k]);
+        wbf[NI * NH + NH * NH + a] = j < NO ? to_bf16(P[OFF_W3 + j * NH + k]) : (unsigned short)0;  // W3t[k][j]
+    }
+    if (a < NH) {
+        bias[a] = P[OFF_B1 + a];
+        bias[NH + a] = P[OFF_B2 + a];
+        if (a < NO) bias[2 * NH + a] = P[OFF_B3 + a];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// bf16 MFMA forward of 64 faces per wavefront
+// v_mfma_f32_32x32x8bf16_1k lane layouts (M=N=32, K=8):
+//   A: lane l holds A[l%32][4*(l/32)+v], v=0..3
+//   B: lane l holds B[4*(l/32)+v][l%32]
+//   D: lane l, register r holds D[8*(r/4)+4*(l/32)+(r%4)][l%32]
+// Each wave stages its 64x40 bf16 inputs, hidden activations and outputs
+// through its own LDS slice (no cross-wave sharing, so only the wave's own
+// s_barrier-free ordering is needed; __syncthreads is used between phases).
+// ---------------------------------------------------------------------------
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+struct WaveLds {
+    unsigned short xa[64][NI + 8];  // inputs (bf16), padded row
+    unsigned short h[64][NH + 8];   // hidden activations (bf16)
+    float y[64][NO + 1];            // outputs
+};
+
+__device__ __forceinline__ s4 ld_a(const unsigned short* row, int k0) {
+    s4 r;
+    r.x = (short)row[k0];
+    r.y = (short)row[k0 + 1];
+    r.z = (short)row[k0 + 2];
+    r.w = (short)row[k0 + 3];
+    return r;
+}
+
+// B operands of one layer for this lane: nk k-blocks of 8
+template <int NK>
+__device__ __forceinline__ void ld_b(const unsigned short* __restrict__ wt, int lane, s4 (&b)[NK]) {
+    const int j = lane & 31, kk = 4 * (lane >> 5);
+#pragma unroll
+    for (int kb = 0; kb < NK; ++kb) {
+        s4 r;
+        r.x = (short)wt[(kb * 8 + kk + 0) * NH + j];
+        r.y = (short)wt[(kb * 8 + kk + 1) * NH + j];
+        r.z = (short)wt[(kb * 8 + kk + 2) * NH + j];
+        r.w = (short)wt[(kb * 8 + kk + 3) * NH + j];
+        b[kb] = r;
+    }
+}
+
+// one 32-row tile: D = A(rows r0..r0+31 of src, K = 8*NK) * B
+template <int NK, int LD>
+__device__ __forceinline__ f16v mfma_tile(const unsigned short (*src)[LD], int r0, int lane, const s4 (&b)[NK]) {
+    f16v acc = {};
+    const int i = lane & 31, kk = 4 * (lane >> 5);
+#pragma unroll
+    for (int kb = 0; kb < NK; ++kb) {
+        const s4 a = ld_a(src[r0 + i], kb * 8 + kk);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b[kb], acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// hidden layer epilogue: bias + activation, bf16 into h (or fp32 into y)
+__device__ __forceinline__ void store_hidden(WaveLds& L, const f16v& acc, int r0, int lane,
+                                             const float* __restrict__ bias) {
+    const int j = lane & 31, ib = 4 * (lane >> 5);
+    const float bj = bias[j];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int i = 8 * (r >> 2) + ib + (r & 3);
+        L.h[r0 + i][j] = to_bf16(act_fast(acc[r] + bj));
+    }
+}
+
+// forward of the 64 faces whose normalised inputs sit in L.xa; y in L.y
+__device__ __forceinline__ void mlp64(WaveLds& L, int lane, const unsigned short* __restrict__ wbf,
+                                      const float* __restrict__ bias) {
+    s4 b1[5], b2[4], b3[4];
+    ld_b<5>(wbf, lane, b1);
+    ld_b<4>(wbf + NI * NH, lane, b2);
+    ld_b<4>(wbf + NI * NH + NH * NH, lane, b3);
+    __syncthreads();
+    f16v h0 = mfma_tile<5>(L.xa, 0, lane, b1);
+    f16v h1 = mfma_tile<5>(L.xa, 32, lane, b1);
+    store_hidden(L, h0, 0, lane, bias);
+    store_hidden(L, h1, 32, lane, bias);
+    __syncthreads();
+    h0 = mfma_tile<4>(L.h, 0, lane, b2);
+    h1 = mfma_tile<4>(L.h, 32, lane, b2);
+    __syncthreads();
+    store_hidden(L, h0, 0, lane, bias + NH);
+    store_hidden(L, h1, 32, lane, bias + NH);
+    __syncthreads();
+    h0 = mfma_tile<4>(L.h, 0, lane, b3);
+    h1 = mfma_tile<4>(L.h, 32, lane, b3);
+    const int j = lane & 31, ib = 4 * (lane >> 5);
+    if (j < NO) {
+        const float bj = bias[2 * NH + j];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = 8 * (r >> 2) + ib + (r & 3);
+            L.y[i][j] = h0[r] + bj;
+            L.y[32 + i][j] = h1[r] + bj;
+        }
+    }
+    __syncthreads();
+}
+
+// forward of host-given normalised inputs (tests): n rows of 40 floats
+__global__ void __launch_bounds__(kBlock) k_infer(const float* __restrict__ x, long n,
+                                                 const unsigned short* __restrict__ wbf,
+                                                 const float* __restrict__ bias, float* __restrict__ y) {
+    __shared__ WaveLds lds[kBlock / 64];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    WaveLds& L = lds[wv];
+    const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int a = 0; a < NI; ++a) L.xa[lane][a] = to_bf16(row < n ? x[row * NI + a] : 0.f);
+    float xr[NI];
+#pragma unroll
+    for (int a = 0; a < NI; ++a) xr[a] = row < n ? x[row * NI + a] : 0.f;
+    mlp64(L, lane, wbf, bias);
+    if (row < n)
+        for (int o = 0; o < NO; ++o)
+            y[row * NO + o] = lin4(xr + 5 * (o & 7)) + rough5(xr + 5 * (o & 7)) * L.y[lane][o];
+}
+
+// ---------------------------------------------------------------------------
+// fp32 face flux with the learned reconstruction (PDM fallback)
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_flux_sur(Dims d, Coef<float> c, Prim<float> q, float theta,
+                                                    const unsigned short* __restrict__ wbf,
+                                                    const float* __restrict__ bias, float* __restrict__ F, long nface,
+                                                    unsigned long long* __restrict__ stats) {
+    __shared__ WaveLds lds[kBlock / 64];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    WaveLds& L = lds[wv];
+    const long t = gtid();
+    const bool live = t < nface;
+    const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
+    int i = 1, j = 1, k = 1;
+    if (live) unflat(t, ii, jj, i, j, k);
+    const ptrdiff_t base = ix(d.nx, d.ny, d.ng + i, d.ng + j, d.ng + k);
+    const ptrdiff_t st = D == 1 ? 1 : (D == 2 ? (ptrdiff_t)d.nx : (ptrdiff_t)d.nx * d.ny);
+    const float* f[8] = {q.rho, q.vx, q.vy, q.vz, q.p, q.bx, q.by, q.bz};
+    float x[NI], cc[8], mm[8], sc[8], ss[8], lin[8], gate[8], rough = 0.f;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        if (live) load5<float>(f[v], base, st, x + 5 * v, cc[v], mm[v]);
+        else {
+#pragma unroll
+            for (int a = 0; a < 5; ++a) x[5 * v + a] = 0.f;
+            cc[v] = mm[v] = 0.f;
+        }
+    }
+    var_scales<float>(cc, sc);
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        ss[v] = norm_scale<float>(mm[v], sc[v]);
+        const float r = __frcp_rn(ss[v]);
+#pragma unroll
+        for (int a = 0; a < 5; ++a) x[5 * v + a] = (x[5 * v + a] - cc[v]) * r;
+        gate[v] = rough5(x + 5 * v);
+        rough = fmaxf(rough, gate[v]);
+        lin[v] = lin4(x + 5 * v);
+#pragma unroll
+        for (int a = 0; a < 5; ++a) L.xa[lane][5 * v + a] = to_bf16(x[5 * v + a]);
+    }
+    mlp64(L, lane, wbf, bias);
+    // the face states, clamped to the range of the two cells next to the
+    // face (PDM's own bound, mhd3d3.f90:1142-1166)
+    float lv[8], rv[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        const float lo = fminf(x[5 * v + 1], x[5 * v + 2]), hi = fmaxf(x[5 * v + 1], x[5 * v + 2]);
+        lv[v] = cc[v] + ss[v] * fminf(hi, fmaxf(lo, lin[v] + gate[v] * L.y[lane][v]));
+        rv[v] = cc[v] + ss[v] * fminf(hi, fmaxf(lo, lin[v] + gate[v] * L.y[lane][8 + v]));
+    }
+    bool fb = !(rough <= theta) || !(lv[0] > 0.f) || !(rv[0] > 0.f) || !(lv[4] > 0.f) || !(rv[4] > 0.f);
+    if (live && fb) {
+#pragma unroll
+        for (int v = 0; v < 8; ++v) recon_line<float, PDM>(f[v], base, st, c, lv[v], rv[v]);
+    }
+    if (stats) {
+        unsigned long long nl = __ballot(live), nf = __ballot(live && fb);
+        if (lane == 0) {
+            atomicAdd(&stats[0], (unsigned long long)__popcll(nl));
+            atomicAdd(&stats[1], (unsigned long long)__popcll(nf));
+        }
+    }
+    if (!live) return;
+    face_flux<float, D>(c, lv, rv, F, t, nface);
+}
+
+}  // namespace sur
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct SurrogateState : sur::State {};
+
+static SurrogateState* sstate(mhd_ctx* c) {
+    if (!c->sur) {
+        SurrogateState* s = new SurrogateState();
+        const size_t pb = sur::NPARAM * sizeof(float);
+        bool ok = hipMalloc(&s->p, pb) == hipSuccess && hipMalloc(&s->g, pb) == hipSuccess &&
+                  hipMalloc(&s->m, pb) == hipSuccess && hipMalloc(&s->v, pb) == hipSuccess &&
+                  hipMalloc(&s->count, 64) == hipSuccess && hipMalloc(&s->loss, 64) == hipSuccess &&
+                  hipMalloc(&s->wbf, (sur::NI * sur::NH + 2 * sur::NH * sur::NH) * 2) == hipSuccess &&
+                  hipMalloc(&s->bias, (2 * sur::NH + sur::NO) * sizeof(float)) == hipSuccess &&
+                  hipMalloc(&s->stats, 64) == hipSuccess;
+        if (!ok) {
+            delete s;
+            return nullptr;
+        }
+        hipMemsetAsync(s->g, 0, pb, c->s);
+        hipMemsetAsync(s->m, 0, pb, c->s);
+        hipMemsetAsync(s->v, 0, pb, c->s);
+        hipMemsetAsync(s->p, 0, pb, c->s);
+        hipMemsetAsync(s->count, 0, 64, c->s);
+        hipMemsetAsync(s->stats, 0, 64, c->s);
+        c->sur = s;
+    }
+    return c->sur;
+}
+
+void surrogate_free(mhd_ctx* c) {
+    SurrogateState* s = c->sur;
+    if (!s) return;
+    void* ps[] = {s->p, s->g, s->m, s->v, s->samples, s->count, s->scratch, s->loss, s->wbf, s->bias, s->stats};
+    for (void* p : ps)
+        if (p) hipFree(p);
+    delete s;
+    c->sur = nullptr;
+}
+
+void launch_flux_surrogate(mhd_ctx* c, Src<float> s) {
+    SurrogateState* S = c->sur;
+    if (!S || !S->have_weights) {  // no trained net: plain fp32 PDM
+        launch_flux_fp32(c, s);
+        return;
+    }
+    const Coef<float> co = make_coef<float>(c);
+    const Prim<float> q{s.rho, s.vx, s.vy, s.vz, s.p, s.bx, s.by, s.bz};
+    const float theta = c->sur_theta;
+    unsigned long long* st = c->sur_stats ? S->stats : nullptr;
+    launch(c, MHD_K_SURR, blocks(c->nf[0]), dim3(kBlock), sur::k_flux_sur<1>, c->d, co, q, theta,
+           (const unsigned short*)S->wbf, (const float*)S->bias, (float*)c->flux[0], c->nf[0], st);
+    launch(c, MHD_K_SURR, blocks(c->nf[1]), dim3(kBlock), sur::k_flux_sur<2>, c->d, co, q, theta,
+           (const unsigned short*)S->wbf, (const float*)S->bias, (float*)c->flux[1], c->nf[1], st);
+    launch(c, MHD_K_SURR, blocks(c->nf[2]), dim3(kBlock), sur::k_flux_sur<3>, c->d, co, q, theta,
+           (const unsigned short*)S->wbf, (const float*)S->bias, (float*)c->flux[2], c->nf[2], st);
+}
+
+static int sync_or(mhd_ctx* c) {
+    if (c->launch_error != hipSuccess) {
+        c->launch_error = hipSuccess;
+        return MHD_ERR_HIP;
+    }
+    return hipStreamSynchronize(c->s) == hipSuccess ? MHD_OK : MHD_ERR_HIP;
+}
+
+int mhd_surrogate_set_weights(mhd_ctx* c, const float* w, int n) {
+    if (!c || !w || n != sur::NPARAM) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    SurrogateState* S = sstate(c);
+    if (!S) return MHD_ERR_HIP;
+    if (hipMemcpyAsync(S->p, w, n * sizeof(float), hipMemcpyHostToDevice, c->s) != hipSuccess) return MHD_ERR_HIP;
+    launch(c, MHD_K_SURR, dim3((sur::NI * sur::NH + 255) / 256), dim3(256), sur::k_pack_bf16, (const float*)S->p,
+           S->wbf, S->bias);
+    S->have_weights = true;
+    return sync_or(c);
+}
+
+int mhd_surrogate_get_weights(mhd_ctx* c, float* w, int n) {
+    if (!c || !w || n != sur::NPARAM || !c->sur) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    if (hipMemcpyAsync(w, c->sur->p, n * sizeof(float), hipMemcpyDeviceToHost, c->s) != hipSuccess)
+        return MHD_ERR_HIP;
+    return sync_or(c);
+}
+
+int mhd_surrogate_configure(mhd_ctx* c, double theta, int count_fallbacks) {
+    if (!c || !(theta >= 0.)) return MHD_ERR_PARAM;
+    c->sur_theta = (float)theta;
+    c->sur_stats = count_fallbacks != 0;
+    return MHD_OK;
+}
+
+int mhd_surrogate_stats(mhd_ctx* c, long* faces, long* fallbacks) {
+    if (!c) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    unsigned long long h[2] = {0, 0};
+    if (c->sur && hipMemcpyAsync(h, c->sur->stats, sizeof(h), hipMemcpyDeviceToHost, c->s) != hipSuccess)
+        return MHD_ERR_HIP;
+    int rc = sync_or(c);
+    if (faces) *faces = (long)h[0];
+    if (fallbacks) *fallbacks = (long)h[1];
+    if (c->sur) hipMemsetAsync(c->sur->stats, 0, sizeof(h), c->s);
+    return rc;
+}
+
+int mhd_surrogate_collect(mhd_ctx* c, double theta, int stride, long capacity, long* collected) {
+    if (!c || stride < 1 || capacity < 1 || !c->have_geometry) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    SurrogateState* S = sstate(c);
+    if (!S) return MHD_ERR_HIP;
+    if (S->cap != capacity) {
+        if (S->samples) hipFree(S->samples);
+        S->samples = nullptr;
+        if (hipMalloc(&S->samples, (size_t)capacity * sur::NS * sizeof(float)) != hipSuccess) return MHD_ERR_HIP;
+        S->cap = capacity;
+        hipMemsetAsync(S->count, 0, 64, c->s);
+    }
+    const Coef<double> co = make_coef<double>(c);
+    const Prim<double> q{c->rho, c->vx, c->vy, c->vz, c->pr, c->bx, c->by, c->bz};
+    const int phase = (int)(S->step++ % stride);
+    launch(c, MHD_K_SURR, blocks(c->nf[0]), dim3(kBlock), sur::k_collect<1>, c->d, co, q, (float)theta, stride,
+           phase, S->samples, S->cap, S->count, c->nf[0]);
+    launch(c, MHD_K_SURR, blocks(c->nf[1]), dim3(kBlock), sur::k_collect<2>, c->d, co, q, (float)theta, stride,
+           phase, S->samples, S->cap, S->count, c->nf[1]);
+    launch(c, MHD_K_SURR, blocks(c->nf[2]), dim3(kBlock), sur::k_collect<3>, c->d, co, q, (float)theta, stride,
+           phase, S->samples, S->cap, S->count, c->nf[2]);
+    long h = 0;
+    if (hipMemcpyAsync(&h, S->count, sizeof(long), hipMemcpyDeviceToHost, c->s) != hipSuccess) return MHD_ERR_HIP;
+    int rc = sync_or(c);
+    if (collected) *collected = h < S->cap ? h : S->cap;
+    return rc;
+}
+
+int mhd_surrogate_load_samples(mhd_ctx* c, const float* samples, long n) {
+    if (!c || !samples || n < 1) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    SurrogateState* S = sstate(c);
+    if (!S) return MHD_ERR_HIP;
+    if (S->cap < n) {
+        if (S->samples) hipFree(S->samples);
+        S->samples = nullptr;
+        if (hipMalloc(&S->samples, (size_t)n * sur::NS * sizeof(float)) != hipSuccess) return MHD_ERR_HIP;
+        S->cap = n;
+    }
+    if (hipMemcpyAsync(S->samples, samples, (size_t)n * sur::NS * sizeof(float), hipMemcpyHostToDevice, c->s) !=
+        hipSuccess)
+        return MHD_ERR_HIP;
+    if (hipMemcpyAsync(S->count, &n, sizeof(long), hipMemcpyHostToDevice, c->s) != hipSuccess) return MHD_ERR_HIP;
+    return sync_or(c);
+}
+
+int mhd_surrogate_fetch_samples(mhd_ctx* c, float* out, long n) {
+    if (!c || !out || !c->sur || n < 1 || n > c->sur->cap) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    if (hipMemcpyAsync(out, c->sur->samples, (size_t)n * sur::NS * sizeof(float), hipMemcpyDeviceToHost, c->s) !=
+        hipSuccess)
+        return MHD_ERR_HIP;
+    return sync_or(c);
+}
+
+static int ensure_scratch(mhd_ctx* c, SurrogateState* S, long batch) {
+    if (S->scratch_n >= batch) return MHD_OK;
+    if (S->scratch) hipFree(S->scratch);
+    S->scratch = nullptr;
+    if (hipMalloc(&S->scratch, (size_t)batch * sur::SC * sizeof(float)) != hipSuccess) return MHD_ERR_HIP;
+    S->scratch_n = batch;
+    return MHD_OK;
+}
+
+static void run_minibatch(mhd_ctx* c, SurrogateState* S, const float* P, const float* set, long n, long first,
+                          long mult, int batch) {
+    launch(c, MHD_K_SURR, blocks(batch), dim3(kBlock), sur::k_train_fb, P, set, n, first, mult, batch,
+           1.f / (float)(batch * sur::NO), S->scratch, S->loss);
+    launch(c, MHD_K_SURR, dim3((sur::NPARAM + 63) / 64, 16), dim3(kBlock), sur::k_train_grad,
+           (const float*)S->scratch, batch, S->g);
+}
+
+static long gcd_l(long a, long b) {
+    while (b) {
+        long t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+int mhd_surrogate_train(mhd_ctx* c, int epochs, int batch, double lr, unsigned seed, double* mean_loss) {
+    if (!c || epochs < 1 || batch < 64 || !(lr > 0.)) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    SurrogateState* S = sstate(c);
+    if (!S) return MHD_ERR_HIP;
+    long n = 0;
+    if (hipMemcpyAsync(&n, S->count, sizeof(long), hipMemcpyDeviceToHost, c->s) != hipSuccess) return MHD_ERR_HIP;
+    if (hipStreamSynchronize(c->s) != hipSuccess) return MHD_ERR_HIP;
+    n = n < S->cap ? n : S->cap;
+    if (n < batch) return MHD_ERR_PARAM;
+    if (ensure_scratch(c, S, batch) != MHD_OK) return MHD_ERR_HIP;
+    if (!S->have_weights) {  // deterministic initialisation: uniform(-1/sqrt(fan_in), 1/sqrt(fan_in))
+        std::vector<float> w(sur::NPARAM, 0.f);
+        unsigned long long z = 0x9E3779B97F4A7C15ull * (seed + 1);
+        auto rnd = [&]() {
+            z += 0x9E3779B97F4A7C15ull;
+            unsigned long long r = z;
+            r = (r ^ (r >> 30)) * 0xBF58476D1CE4E5B9ull;
+            r = (r ^ (r >> 27)) * 0x94D049BB133111EBull;
+            r ^= r >> 31;
+            return (double)(r >> 11) * (1.0 / 9007199254740992.0);
+        };
+        auto fill = [&](int off, int cnt, int fan) {
+            const double a = 1. / std::sqrt((double)fan);
+            for (int q = 0; q < cnt; ++q) w[off + q] = (float)((2. * rnd() - 1.) * a);
+        };
+        fill(sur::OFF_W1, sur::NH * sur::NI, sur::NI);
+        fill(sur::OFF_W2, sur::NH * sur::NH, sur::NH);
+        fill(sur::OFF_W3, sur::NO * sur::NH, sur::NH);
+        // the last layer starts small: the net starts at the linear part
+        for (int q = 0; q < sur::NO * sur::NH; ++q) w[sur::OFF_W3 + q] *= .01f;
+        if (hipMemcpyAsync(S->p, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice, c->s) != hipSuccess)
+            return MHD_ERR_HIP;
+        S->have_weights = true;
+    }
+    const long nb = n / batch;
+    double tot = 0.;
+    unsigned long long z = 0xD1B54A32D192ED03ull ^ seed;
+    long adam_t = 0;
+    std::vector<float> lsum(1);
+    for (int e = 0; e < epochs; ++e) {
+        // a fresh permutation per epoch: sample b -> (first + b*mult) mod n
+        z = z * 6364136223846793005ull + 1442695040888963407ull;
+        long mult = (long)((z >> 17) % (unsigned long long)n) | 1;
+        while (gcd_l(mult, n) != 1) mult += 2;
+        const long first = (long)((z >> 7) % (unsigned long long)n);
+        launch(c, MHD_K_SURR, dim3(1), dim3(64), sur::k_zero_f, S->loss, 1);
+        for (long bi = 0; bi < nb; ++bi) {
+            run_minibatch(c, S, S->p, S->samples, n, (first + bi * batch * (mult % n)) % n, mult, batch);
+            ++adam_t;
+            const double b1c = 1. - std::pow(.9, (double)adam_t), b2c = 1. - std::pow(.999, (double)adam_t);
+            const double prog = ((double)e + (double)bi / (double)nb) / (double)epochs;
+            const double lr_t = lr * .5 * (1. + std::cos(3.141592653589793 * prog));
+            launch(c, MHD_K_SURR, dim3((sur::NPARAM + 255) / 256), dim3(256), sur::k_adam, S->p, S->g, S->m, S->v,
+                   (float)lr_t, (float)b1c, (float)b2c);
+        }
+        float h = 0.f;
+        if (hipMemcpyAsync(&h, S->loss, sizeof(float), hipMemcpyDeviceToHost, c->s) != hipSuccess) return MHD_ERR_HIP;
+        if (hipStreamSynchronize(c->s) != hipSuccess) return MHD_ERR_HIP;
+        tot = (double)h / (double)(nb * batch * sur::NO);
+    }
+    launch(c, MHD_K_SURR, dim3((sur::NI * sur::NH + 255) / 256), dim3(256), sur::k_pack_bf16, (const float*)S->p,
+           S->wbf, S->bias);
+    if (mean_loss) *mean_loss = tot;
+    return sync_or(c);
+}
+
+int mhd_surrogate_grad(mhd_ctx* c, const float* w, const float* samples, int n, double* loss, float* grad) {
+    if (!c || !w || !samples || !grad || n < 1) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    SurrogateState* S = sstate(c);
+    if (!S) return MHD_ERR_HIP;
+    if (ensure_scratch(c, S, n) != MHD_OK) return MHD_ERR_HIP;
+    // the given weights go to a buffer of their own: the context's net, its
+    // Adam moments and its gradient accumulator are left as they are
+    float* dset = nullptr;
+    if (hipMalloc(&dset, ((size_t)n * sur::NS + 2 * sur::NPARAM) * sizeof(float)) != hipSuccess) return MHD_ERR_HIP;
+    float* dw = dset + (size_t)n * sur::NS;
+    float* dg = dw + sur::NPARAM;
+    float* g_keep = S->g;
+    S->g = dg;
+    int rc = MHD_OK;
+    if (hipMemcpyAsync(dw, w, sur::NPARAM * sizeof(float), hipMemcpyHostToDevice, c->s) != hipSuccess ||
+        hipMemcpyAsync(dset, samples, (size_t)n * sur::NS * sizeof(float), hipMemcpyHostToDevice, c->s) != hipSuccess)
+        rc = MHD_ERR_HIP;
+    if (rc == MHD_OK) {
+        launch(c, MHD_K_SURR, dim3(1), dim3(64), sur::k_zero_f, S->loss, 1);
+        launch(c, MHD_K_SURR, dim3((sur::NPARAM + 255) / 256), dim3(256), sur::k_zero_f, dg, sur::NPARAM);
+        run_minibatch(c, S, dw, dset, n, 0, 1, n);
+        float h = 0.f;
+        if (hipMemcpyAsync(grad, dg, sur::NPARAM * sizeof(float), hipMemcpyDeviceToHost, c->s) != hipSuccess ||
+            hipMemcpyAsync(&h, S->loss, sizeof(float), hipMemcpyDeviceToHost, c->s) != hipSuccess)
+            rc = MHD_ERR_HIP;
+        int r2 = sync_or(c);
+        if (rc == MHD_OK) rc = r2;
+        if (loss) *loss = (double)h / (double)(n * sur::NO);
+    }
+    S->g = g_keep;
+    hipFree(dset);
+    return rc;
+}
+
+int mhd_surrogate_infer(mhd_ctx* c, const float* x, long n, float* y) {
+    if (!c || !x || !y || n < 1 || !c->sur || !c->sur->have_weights) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    SurrogateState* S = c->sur;
+    float *dx = nullptr, *dy = nullptr;
+    if (hipMalloc(&dx, (size_t)n * sur::NI * sizeof(float)) != hipSuccess) return MHD_ERR_HIP;
+    if (hipMalloc(&dy, (size_t)n * sur::NO * sizeof(float)) != hipSuccess) {
+        hipFree(dx);
+        return MHD_ERR_HIP;
+    }
+    int rc = MHD_OK;
+    if (hipMemcpyAsync(dx, x, (size_t)n * sur::NI * sizeof(float), hipMemcpyHostToDevice, c->s) != hipSuccess)
+        rc = MHD_ERR_HIP;
+    if (rc == MHD_OK) {
+        launch(c, MHD_K_SURR, blocks(n), dim3(kBlock), sur::k_infer, (const float*)dx, n,
+               (const unsigned short*)S->wbf, (const float*)S->bias, dy);
+        if (hipMemcpyAsync(y, dy, (size_t)n * sur::NO * sizeof(float), hipMemcpyDeviceToHost, c->s) != hipSuccess)
+            rc = MHD_ERR_HIP;
+        int r2 = sync_or(c);
+        if (rc == MHD_OK) rc = r2;
+    }
+    hipFree(dx);
+    hipFree(dy);
+    return rc;
+}

@@ -37,6 +37,7 @@ class Params(C.Structure):
 
 
 _D = C.POINTER(C.c_double)
+_F = C.POINTER(C.c_float)
 _lib = None
 
 # every entry point of include/mhd_gpu.h with its ctypes signature
@@ -63,6 +64,16 @@ SIGNATURES = {
     "mhd_device_pointer": (C.c_void_p, [C.c_void_p, C.c_int]),
     "mhd_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "mhd_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]),
+    "mhd_surrogate_set_weights": (C.c_int, [C.c_void_p, _F, C.c_int]),
+    "mhd_surrogate_get_weights": (C.c_int, [C.c_void_p, _F, C.c_int]),
+    "mhd_surrogate_configure": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "mhd_surrogate_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+    "mhd_surrogate_collect": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_long, C.POINTER(C.c_long)]),
+    "mhd_surrogate_load_samples": (C.c_int, [C.c_void_p, _F, C.c_long]),
+    "mhd_surrogate_fetch_samples": (C.c_int, [C.c_void_p, _F, C.c_long]),
+    "mhd_surrogate_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_uint, _D]),
+    "mhd_surrogate_grad": (C.c_int, [C.c_void_p, _F, _F, C.c_int, _D, _F]),
+    "mhd_surrogate_infer": (C.c_int, [C.c_void_p, _F, C.c_long, _F]),
 }
 
 

@@ -59,6 +59,11 @@ struct orc_state {
     A3 rhop, vxp, vyp, vzp, pp, bxp, byp, bzp, bip, bjp, bkp;
     A3 rhoh, vxh, vyh, vzh, ph, bxh, byh, bzh, bih, bjh, bkh;
     A3 rhovxp, rhovyp, rhovzp, rhovxh, rhovyh, rhovzh, engp;
+    /* learned reconstruction in the flux (orc_set_surrogate) */
+    float *sur_w;
+    double sur_theta;
+    int sur_round;
+    long sur_faces, sur_fallbacks;
 };
 
 static const double PI = 3.141592653589793238462643383279502884; /* = 4*atan(1) in binary64 */
@@ -408,6 +413,8 @@ static void eijk_lfm(orc_state *s, const A3 *lrho, const A3 *lvx, const A3 *lvy,
 /* --------------------------------------------------------------------- */
 /* get_flux_stress (mhd3d3.f90:2385-2560)                                 */
 /* --------------------------------------------------------------------- */
+static int sur_face(const orc_state *s, const A3 *const *f, int dir, int i, int j, int k, double *lv,
+                   double *rv);
 static void flux_stress(orc_state *s, const A3 *lrho, const A3 *lvx, const A3 *lvy, const A3 *lvz, const A3 *lp,
                         const A3 *lbx, const A3 *lby, const A3 *lbz) {
     const orc_params *p = &s->p;
@@ -420,14 +427,25 @@ static void flux_stress(orc_state *s, const A3 *lrho, const A3 *lvx, const A3 *l
             for (int j = 1; j <= jj; ++j)
                 for (int i = 1; i <= ii; ++i) {
                     double rl, rr, vxl, vxr, vyl, vyr, vzl, vzr, pl, prr, bxl, bxr, byl, byr, bzl, bzr;
-                    recon_pt(p, lrho, kkk, i, j, k, &rl, &rr);
-                    recon_pt(p, lvx, kkk, i, j, k, &vxl, &vxr);
-                    recon_pt(p, lvy, kkk, i, j, k, &vyl, &vyr);
-                    recon_pt(p, lvz, kkk, i, j, k, &vzl, &vzr);
-                    recon_pt(p, lp, kkk, i, j, k, &pl, &prr);
-                    recon_pt(p, lbx, kkk, i, j, k, &bxl, &bxr);
-                    recon_pt(p, lby, kkk, i, j, k, &byl, &byr);
-                    recon_pt(p, lbz, kkk, i, j, k, &bzl, &bzr);
+                    int net = 0;
+                    if (s->sur_w) {
+                        const A3 *fv[8] = {lrho, lvx, lvy, lvz, lp, lbx, lby, lbz};
+                        double lv[8], rv[8];
+                        net = sur_face(s, fv, kkk, i, j, k, lv, rv);
+                        rl = lv[0], rr = rv[0], vxl = lv[1], vxr = rv[1], vyl = lv[2], vyr = rv[2];
+                        vzl = lv[3], vzr = rv[3], pl = lv[4], prr = rv[4], bxl = lv[5], bxr = rv[5];
+                        byl = lv[6], byr = rv[6], bzl = lv[7], bzr = rv[7];
+                    }
+                    if (!net) {
+                        recon_pt(p, lrho, kkk, i, j, k, &rl, &rr);
+                        recon_pt(p, lvx, kkk, i, j, k, &vxl, &vxr);
+                        recon_pt(p, lvy, kkk, i, j, k, &vyl, &vyr);
+                        recon_pt(p, lvz, kkk, i, j, k, &vzl, &vzr);
+                        recon_pt(p, lp, kkk, i, j, k, &pl, &prr);
+                        recon_pt(p, lbx, kkk, i, j, k, &bxl, &bxr);
+                        recon_pt(p, lby, kkk, i, j, k, &byl, &byr);
+                        recon_pt(p, lbz, kkk, i, j, k, &bzl, &bzr);
+                    }
                     double v0p, v0n, v1p, v1n;
                     /* left */
                     double leng = .5 * rl * (sq(vxl) + sq(vyl) + sq(vzl)) + pl / (gam - 1.);
@@ -853,6 +871,7 @@ void orc_destroy(orc_state *s) {
     free(s->dx);
     free(s->dy);
     free(s->dz);
+    free(s->sur_w);
     free(s);
 }
 
@@ -943,4 +962,196 @@ void orc_set_threads(int n) {
 #else
     (void)n;
 #endif
+}
+
+/* --------------------------------------------------------------------- */
+/* learned reconstruction: CPU restatement of the net (checker for the   */
+/* GPU training and inference kernels).  No reference counterpart.       */
+/* y = lin(x) + gate(x) * MLP(x), per output of variable q:             */
+/* lin = (-x0 + 7x1 + 7x2 - x3)/12, gate = max |x(k-1) - 2x(k) + x(k+1)| */
+/* of q's window                                                          */
+/* layout: W1[32][40] b1[32] W2[32][32] b2[32] W3[16][32] b3[16]         */
+/* activation g(z) = z*sigmoid(1.702 z); loss = mean squared error over  */
+/* the 16 outputs of the n samples (56 floats: 40 inputs, 16 targets)    */
+/* --------------------------------------------------------------------- */
+#define MNI 40
+#define MNH 32
+#define MNO 16
+#define MOFF_B1 (MNH * MNI)
+#define MOFF_W2 (MOFF_B1 + MNH)
+#define MOFF_B2 (MOFF_W2 + MNH * MNH)
+#define MOFF_W3 (MOFF_B2 + MNH)
+#define MOFF_B3 (MOFF_W3 + MNO * MNH)
+#define MNPARAM (MOFF_B3 + MNO)
+
+static double mact(double z) { return z / (1. + exp(-1.702 * z)); }
+/* fixed linear part: 4th-order interpolation of the window of output o */
+static double mlin(const float *x, int o) {
+    const float *w = x + 5 * (o & 7);
+    return (-(double)w[0] + 7. * w[1] + 7. * w[2] - w[3]) / 12.;
+}
+/* gate of output o: largest |second difference| of its variable's window,
+ * so the learned correction vanishes on linear data */
+static double mgate(const float *x, int o) {
+    const float *w = x + 5 * (o & 7);
+    double g = 0.;
+    for (int a = 1; a < 4; ++a) g = fmax(g, fabs((double)w[a - 1] - 2. * w[a] + w[a + 1]));
+    return g;
+}
+static double mact_grad(double z) {
+    double s = 1. / (1. + exp(-1.702 * z));
+    return s + 1.702 * z * s * (1. - s);
+}
+
+/* forward of n rows of 40 inputs; h_round != 0 rounds the hidden
+ * activations to bf16 (round to nearest even) like the MFMA inference */
+static double bf16r(double v) {
+    float f = (float)v;
+    unsigned int u;
+    memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+void orc_mlp_forward(const float *w, const float *x, long n, int h_round, float *y) {
+    for (long s = 0; s < n; ++s) {
+        const float *xi = x + s * MNI;
+        double a1[MNH], a2[MNH];
+        for (int j = 0; j < MNH; ++j) {
+            double acc = 0.;
+            for (int a = 0; a < MNI; ++a) acc += (double)w[j * MNI + a] * xi[a];
+            acc += w[MOFF_B1 + j];
+            a1[j] = h_round ? bf16r(mact(acc)) : mact(acc);
+        }
+        for (int j = 0; j < MNH; ++j) {
+            double acc = 0.;
+            for (int a = 0; a < MNH; ++a) acc += (double)w[MOFF_W2 + j * MNH + a] * a1[a];
+            acc += w[MOFF_B2 + j];
+            a2[j] = h_round ? bf16r(mact(acc)) : mact(acc);
+        }
+        for (int o = 0; o < MNO; ++o) {
+            double acc = 0.;
+            for (int a = 0; a < MNH; ++a) acc += (double)w[MOFF_W3 + o * MNH + a] * a2[a];
+            y[s * MNO + o] = (float)(mgate(xi, o) * (acc + w[MOFF_B3 + o]) + mlin(xi, o));
+        }
+    }
+}
+
+/* the learned face states of one face (the solver's use of the net):
+ * windows of cells j-2..j+2 centred on c = (f(j-1)+f(j))/2, divided by
+ * max(range, 1e-3 * physical scale), net y = lin + MLP, clamped to the two
+ * cells' range; returns 0 (PDM fallback) when the largest normalised second
+ * difference exceeds theta or a density/pressure state is not positive.
+ * With sur_round the weights, inputs and hidden activations are rounded to
+ * bf16 as in the MFMA inference. */
+static int sur_face(const orc_state *s, const A3 *const *f, int dir, int i, int j, int k, double *lv,
+                    double *rv) {
+    const int ng = s->p.ng;
+    double w5[8][5], c[8], m[8], sc[8];
+    float xf[MNI];
+    for (int v = 0; v < 8; ++v) {
+        for (int a = 0; a < 5; ++a) w5[v][a] = gat(f[v], ng, dir, i, j, k, a - 2, 0, 0, 0);
+        c[v] = .5 * (w5[v][1] + w5[v][2]);
+        m[v] = 0.;
+        for (int a = 0; a < 5; ++a) m[v] = fmax(m[v], fabs(w5[v][a] - c[v]));
+    }
+    {
+        const double r = fabs(c[0]), pp = fabs(c[4]);
+        const double vs = fmax(fmax(fabs(c[1]), fabs(c[2])), fabs(c[3])) + sqrt(pp / fmax(r, 1e-30));
+        const double bs = fmax(fmax(fabs(c[5]), fabs(c[6])), fabs(c[7])) + sqrt(pp);
+        sc[0] = r, sc[1] = sc[2] = sc[3] = vs, sc[4] = pp, sc[5] = sc[6] = sc[7] = bs;
+    }
+    double ss[8], rough = 0.;
+    for (int v = 0; v < 8; ++v) {
+        ss[v] = fmax(fmax(m[v], 1e-3 * sc[v]), 1e-30);
+        for (int a = 0; a < 5; ++a) xf[5 * v + a] = (float)((w5[v][a] - c[v]) / ss[v]);
+        for (int a = 1; a < 4; ++a)
+            rough = fmax(rough, fabs((double)xf[5 * v + a - 1] - 2. * xf[5 * v + a] + xf[5 * v + a + 1]));
+    }
+    float y[MNO];
+    if (s->sur_round) {
+        float xr[MNI];
+        for (int a = 0; a < MNI; ++a) xr[a] = (float)bf16r(xf[a]);
+        orc_mlp_forward(s->sur_w, xr, 1, 1, y);
+        /* the fixed linear part and the gate use the unrounded inputs */
+        for (int o = 0; o < MNO; ++o) {
+            const double gr = mgate(xr, o), gf = mgate(xf, o);
+            const double corr = gr > 0. ? (y[o] - mlin(xr, o)) / gr : 0.;
+            y[o] = (float)(mlin(xf, o) + gf * corr);
+        }
+    } else {
+        orc_mlp_forward(s->sur_w, xf, 1, 0, y);
+    }
+    for (int v = 0; v < 8; ++v) {
+        const double lo = fmin(xf[5 * v + 1], xf[5 * v + 2]), hi = fmax(xf[5 * v + 1], xf[5 * v + 2]);
+        lv[v] = c[v] + ss[v] * fmin(hi, fmax(lo, (double)y[v]));
+        rv[v] = c[v] + ss[v] * fmin(hi, fmax(lo, (double)y[8 + v]));
+    }
+    return rough <= s->sur_theta && lv[0] > 0. && rv[0] > 0. && lv[4] > 0. && rv[4] > 0.;
+}
+
+void orc_set_surrogate(orc_state *s, const float *w, double theta, int bf16_round) {
+    free(s->sur_w);
+    s->sur_w = NULL;
+    if (!w) return;
+    s->sur_w = (float *)malloc(MNPARAM * sizeof(float));
+    memcpy(s->sur_w, w, MNPARAM * sizeof(float));
+    if (bf16_round)
+        for (int q = 0; q < MNPARAM; ++q) s->sur_w[q] = (float)bf16r(s->sur_w[q]);
+    s->sur_theta = theta;
+    s->sur_round = bf16_round;
+}
+
+/* loss and gradient with respect to the parameters (double accumulation) */
+double orc_mlp_loss_grad(const float *w, const float *samples, long n, double *grad) {
+    double loss = 0.;
+    for (int q = 0; q < MNPARAM; ++q) grad[q] = 0.;
+    const double inv = 1. / ((double)n * MNO);
+    for (long s = 0; s < n; ++s) {
+        const float *x = samples + s * (MNI + MNO), *t = x + MNI;
+        double z1[MNH], a1[MNH], z2[MNH], a2[MNH], dy[MNO], dz2[MNH], dz1[MNH];
+        for (int j = 0; j < MNH; ++j) {
+            double acc = w[MOFF_B1 + j];
+            for (int a = 0; a < MNI; ++a) acc += (double)w[j * MNI + a] * x[a];
+            z1[j] = acc;
+            a1[j] = mact(acc);
+        }
+        for (int j = 0; j < MNH; ++j) {
+            double acc = w[MOFF_B2 + j];
+            for (int a = 0; a < MNH; ++a) acc += (double)w[MOFF_W2 + j * MNH + a] * a1[a];
+            z2[j] = acc;
+            a2[j] = mact(acc);
+        }
+        for (int o = 0; o < MNO; ++o) {
+            double acc = w[MOFF_B3 + o];
+            for (int a = 0; a < MNH; ++a) acc += (double)w[MOFF_W3 + o * MNH + a] * a2[a];
+            const double gt = mgate(x, o);
+            double e = gt * acc + mlin(x, o) - t[o];
+            loss += e * e;
+            dy[o] = 2. * e * gt * inv;
+        }
+        for (int a = 0; a < MNH; ++a) {
+            double acc = 0.;
+            for (int o = 0; o < MNO; ++o) acc += (double)w[MOFF_W3 + o * MNH + a] * dy[o];
+            dz2[a] = acc * mact_grad(z2[a]);
+        }
+        for (int a = 0; a < MNH; ++a) {
+            double acc = 0.;
+            for (int j = 0; j < MNH; ++j) acc += (double)w[MOFF_W2 + j * MNH + a] * dz2[j];
+            dz1[a] = acc * mact_grad(z1[a]);
+        }
+        for (int j = 0; j < MNH; ++j) {
+            for (int a = 0; a < MNI; ++a) grad[j * MNI + a] += dz1[j] * x[a];
+            grad[MOFF_B1 + j] += dz1[j];
+            for (int a = 0; a < MNH; ++a) grad[MOFF_W2 + j * MNH + a] += dz2[j] * a1[a];
+            grad[MOFF_B2 + j] += dz2[j];
+        }
+        for (int o = 0; o < MNO; ++o) {
+            for (int a = 0; a < MNH; ++a) grad[MOFF_W3 + o * MNH + a] += dy[o] * a2[a];
+            grad[MOFF_B3 + o] += dy[o];
+        }
+    }
+    return loss * inv;
 }

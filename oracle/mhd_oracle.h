@@ -57,6 +57,15 @@ void orc_reconstruct(const orc_params *p, const double *f, int fn1, int fn2, int
 /* get_v_face (mhd3d3.f90:2594) elementwise. */
 void orc_v_face(int n, const double *lam, const double *v, double *v0p, double *v0n,
                 double *v1p, double *v1n);
+/* learned reconstruction (MLP 40-32-32-16, activation z*sigmoid(1.702 z)):
+ * forward (h_round: bf16-round hidden activations) and loss/gradient */
+void orc_mlp_forward(const float *w, const float *x, long n, int h_round, float *y);
+double orc_mlp_loss_grad(const float *w, const float *samples, long n, double *grad);
+/* use the net for the face states of the flux (PDM fallback; reconop must
+ * be 1); w = NULL switches it off.  bf16_round: round weights, inputs and
+ * hidden activations to bf16 like the GPU's MFMA inference. */
+void orc_set_surrogate(orc_state *s, const float *w, double theta, int bf16_round);
+
 /* set the number of OpenMP threads used by the oracle (0 = runtime default). */
 void orc_set_threads(int n);
 

@@ -59,6 +59,11 @@ def lib():
                                       C.c_int, C.c_int, _D, _D]
         L.orc_v_face.argtypes = [C.c_int, _D, _D, _D, _D, _D, _D]
         L.orc_set_threads.argtypes = [C.c_int]
+        _F = C.POINTER(C.c_float)
+        L.orc_mlp_forward.argtypes = [_F, _F, C.c_long, C.c_int, _F]
+        L.orc_mlp_loss_grad.argtypes = [_F, _F, C.c_long, _D]
+        L.orc_set_surrogate.argtypes = [C.c_void_p, _F, C.c_double, C.c_int]
+        L.orc_mlp_loss_grad.restype = C.c_double
         _lib = L
     return _lib
 
@@ -147,6 +152,14 @@ class Oracle:
             off += n
         return dict(ei=ei, ej=ej, ek=ek, flux=flux)
 
+    def set_surrogate(self, w, theta=1.0, bf16_round=False):
+        """learned face states in the flux (PDM fallback); w=None: plain PDM"""
+        if w is None:
+            lib().orc_set_surrogate(self.h, None, 0., 0)
+            return
+        self._sur_w = np.ascontiguousarray(w, dtype=np.float32)
+        lib().orc_set_surrogate(self.h, _pf(self._sur_w), float(theta), 1 if bf16_round else 0)
+
     def step(self, nsteps, tau):
         t = C.c_double(tau)
         tc = C.c_double(0.)
@@ -171,3 +184,29 @@ def v_face(lam, v):
     outs = [np.zeros_like(lam) for _ in range(4)]
     lib().orc_v_face(lam.size, _p(lam), _p(v), *[_p(o) for o in outs])
     return outs
+
+
+NPARAM = 2896
+
+
+def _pf(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def mlp_forward(w, x, h_round=False):
+    """MLP forward of rows of 40 inputs (float32 in, float32 out)."""
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n = x.shape[0]
+    y = np.zeros((n, 16), dtype=np.float32)
+    lib().orc_mlp_forward(_pf(w), _pf(x), n, 1 if h_round else 0, _pf(y))
+    return y
+
+
+def mlp_loss_grad(w, samples):
+    """mean squared error and its gradient (double accumulation)."""
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    s = np.ascontiguousarray(samples, dtype=np.float32)
+    g = np.zeros(NPARAM)
+    loss = lib().orc_mlp_loss_grad(_pf(w), _pf(s), s.shape[0], _p(g))
+    return loss, g
