@@ -11,10 +11,15 @@ Two runs from the same initial state:
   * plain: binary64 everywhere (parity-checked against the CPU oracle);
   * fast : fp32 reconstruction/flux/EMF with binary64 conserved
            accumulation and CT face update, plus the learned face
-           reconstruction when trained weights are available.
+           reconstruction (bf16 MFMA, PDM fallback).  The net is trained
+           first, outside the timed region, on face samples of the binary64
+           solver at 128^3 (Orszag-Tang, Gaussian blobs and blast set-ups with
+           seeds distinct from this workload's), by the library's own GPU
+           training kernels.
 `value` is the fast run's whole-job zone-updates/s; the plain run's figure,
-the speed-up and the density L1 difference between the two are reported
-beside it.  A zone-update is one interior cell advanced one full step.
+the speed-up and the density L1 difference between the two after the timed
+steps are reported beside it.  A zone-update is one interior cell advanced
+one full step.
 
 cpu_baseline: the reference solver (oracle/_ref, built from mhd3d3.f90) on
 the host cores with MPI, on a bounded sample of the same workload, timed by
@@ -36,6 +41,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from mhd3d_mpi_fortran_amd import gpu, ics  # noqa: E402
+from mhd3d_mpi_fortran_amd import surrogate as sg  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # algorithmic (compulsory) bytes per output point of each kernel class:
@@ -45,6 +51,11 @@ HBM_PEAK_GBS = 8000.0
 #   flux : 8 primitives read, 8 face products written
 ALG_BYTES = {"emf": {"fp64": 5 * 8 + 8, "mixed": 5 * 4 + 4},
              "flux": {"fp64": 8 * 8 + 8 * 8, "mixed": 8 * 4 + 8 * 4}}
+# compulsory state traffic of one Adams-Bashforth step per zone: read U^n and
+# U^(n-1), write U^(n+1), 8 binary64 words each (rho, rho v, p, three faces)
+AB2_STATE_BYTES = 3 * 8 * 8
+BF16_PEAK_TFLOPS = 2500.0
+MLP_FLOPS = 2 * (40 * 32 + 32 * 32 + 32 * 16)
 
 
 def log(*a):
@@ -57,13 +68,21 @@ def workload(n):
     return g, st
 
 
-def run_gpu(g, st, n, precision, surrogate, warmup, steps):
+def run_gpu(g, st, n, precision, weights, warmup, steps):
     s = gpu.Solver(gpu.make_params(n, n, n, ng=4, reconop=1, stepop=1, bc="periodic", precision=precision,
-                                   surrogate=surrogate), g)
+                                   surrogate=1 if weights is not None else 0), g)
     s.upload(st)
     tau = s.initialize(1e-4)
+    sur = None
+    if weights is not None:
+        sg.set_weights(s, weights)
+        sg.configure(s, sg.THETA, count=True)   # count fallbacks over the warm-up steps only
     if warmup:
         tau, _ = s.step(warmup, tau)
+    if weights is not None:
+        faces, fb = sg.stats(s)
+        sur = dict(faces=faces, fallbacks=fb)
+        sg.configure(s, sg.THETA, count=False)
     s.synchronize()
     s.profile(True)
     t0 = time.perf_counter()
@@ -77,7 +96,7 @@ def run_gpu(g, st, n, precision, surrogate, warmup, steps):
     s.close()
     units = {"emf": [n * (n + 1) * (n + 1), (n + 1) * n * (n + 1), (n + 1) * (n + 1) * n],
              "flux": [(n + 1) * n * n, n * (n + 1) * n, n * n * (n + 1)]}
-    return dict(wall=wall, tau=tau, prof=prof, flags=flags, state=st_out, units=units)
+    return dict(wall=wall, tau=tau, prof=prof, flags=flags, state=st_out, units=units, sur=sur)
 
 
 def l1_rel(a, b, g):
@@ -161,27 +180,48 @@ def cpu_port(g, st, n, nsteps, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--only", choices=["plain", "fast", "both"], default="both")
+    ap.add_argument("--no-surrogate", action="store_true", help="fast run without the learned reconstruction")
+    ap.add_argument("--weights", help="float32 file of trained weights (skips training)")
+    ap.add_argument("--train-n", type=int, default=128)
+    ap.add_argument("--train-epochs", type=int, default=2)
+    ap.add_argument("--theta", type=float, default=None, help="smoothness threshold of the learned reconstruction")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the MPI decomposition of the reference is replaced by a single "
               "whole-domain block; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
     n = args.n
+    if args.theta is not None:
+        sg.THETA = args.theta
     g, st = workload(n)
-    sur = 1 if os.path.exists(os.path.join(ROOT, "mhd3d_mpi_fortran_amd", "weights", "surrogate_pdm.bin")) else 0
+    weights, train_info = None, None
+    if args.only in ("fast", "both") and not args.no_surrogate:
+        if args.weights:
+            weights = np.fromfile(args.weights, dtype=np.float32)
+            train_info = {"weights": os.path.basename(args.weights)}
+        else:
+            log("training the learned reconstruction at %d^3 ..." % args.train_n)
+            t0 = time.perf_counter()
+            weights, loss, nsamp = sg.train_surrogate(n=args.train_n, capacity=1 << 24, epochs=args.train_epochs,
+                                                      log=log)
+            train_info = {"samples": nsamp, "epochs": args.train_epochs, "batch": 1024,
+                          "final_mean_loss": loss, "collect_and_train_s": time.perf_counter() - t0,
+                          "sample_grid": "%d^3" % args.train_n}
+            log("trained on %d samples in %.1f s, loss %.3e" % (nsamp, train_info["collect_and_train_s"], loss))
+    sur = weights is not None
     res = {}
     if args.only in ("plain", "both"):
         log("plain fp64 run ...")
-        res["plain"] = run_gpu(g, st, n, "fp64", 0, args.warmup, args.steps)
+        res["plain"] = run_gpu(g, st, n, "fp64", None, args.warmup, args.steps)
     if args.only in ("fast", "both"):
         log("fast run (mixed precision%s) ..." % (" + learned reconstruction" if sur else ""))
-        res["fast"] = run_gpu(g, st, n, "mixed", sur, args.warmup, args.steps)
+        res["fast"] = run_gpu(g, st, n, "mixed", weights, args.warmup, args.steps)
     head = res.get("fast") or res["plain"]
     head_name = "fast" if "fast" in res else "plain"
     prec = "mixed" if head_name == "fast" else "fp64"
@@ -225,6 +265,18 @@ def main():
         out["density_L1_vs_plain_fp64"] = l1_rel(res["fast"]["state"], res["plain"]["state"], g)
         out["max_divB"] = max_divb(res["fast"]["state"], g)
     out["kernel_ms_per_step"] = {k: v[1] / K for k, v in head["prof"].items() if v[0]}
+    out["step_roofline"] = {"alg_bytes_per_zone_update": AB2_STATE_BYTES,
+                            "achieved_GBs": value * AB2_STATE_BYTES / 1e9,
+                            "frac": value * AB2_STATE_BYTES / 1e9 / HBM_PEAK_GBS}
+    if sur and head_name == "fast":
+        s = head["sur"]
+        faces_per_step = sum(head["units"]["flux"])
+        fl_ms = head["prof"]["flux"][1] / K
+        out["surrogate"] = {"theta": sg.THETA, "fallback_fraction": s["fallbacks"] / max(s["faces"], 1),
+                            "faces_counted": s["faces"], "training": train_info,
+                            "mlp_flops_per_face": MLP_FLOPS,
+                            "mlp_tflops_in_flux_kernels": faces_per_step * MLP_FLOPS / (fl_ms * 1e-3) / 1e12,
+                            "bf16_peak_tflops": BF16_PEAK_TFLOPS}
     if not args.no_cpu_baseline:
         cores = host_cores()
         ranks = 16 if cores >= 16 else 8

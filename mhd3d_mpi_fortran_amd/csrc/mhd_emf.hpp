@@ -164,10 +164,10 @@ __global__ void __launch_bounds__(kBlock) k_emf_march(Dims d, Coef<R> c, EmfIn<T
     const int ii = d.nx0 + (K != 1);
     const int no = K == 3 ? d.nz0 : (K == 1 ? d.ny0 + 1 : d.ny0);
     const int nm = K == 3 ? d.ny0 + 1 : d.nz0 + 1;
-    long r = t / ii;
-    const int i = (int)(t - r * ii) + 1;
-    const int mb = (int)(r / no);
-    const int o = (int)(r - (long)mb * no) + 1;
+    const unsigned r = (unsigned)t / (unsigned)ii;   // index spaces stay below 2^31
+    const int i = (int)((unsigned)t - r * (unsigned)ii) + 1;
+    const int mb = (int)(r / (unsigned)no);
+    const int o = (int)(r - (unsigned)mb * (unsigned)no) + 1;
     const int m0 = mb * T + 1;
     const int cnt = nm - m0 + 1 < T ? nm - m0 + 1 : T;
     // first edge of this thread

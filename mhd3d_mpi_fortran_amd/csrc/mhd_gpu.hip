@@ -60,6 +60,9 @@ int mhd_create(mhd_ctx** out, const mhd_params* p) {
     if (p->ng < min_ghosts(p->reconop)) return MHD_ERR_PARAM;
     // the ghost fill copies from owned cells only: each axis needs nx0 >= ng
     if (p->nx0 < p->ng || p->ny0 < p->ng || p->nz0 < p->ng) return MHD_ERR_PARAM;
+    // index spaces (ghost-inclusive faces and edges) must stay below 2^31
+    if ((double)(p->nx0 + 2 * p->ng + 1) * (p->ny0 + 2 * p->ng + 1) * (p->nz0 + 2 * p->ng + 1) >= 2147483647.)
+        return MHD_ERR_PARAM;
     for (int a = 0; a < 3; ++a)
         if (p->bc[a] != MHD_BC_EVEN && p->bc[a] != MHD_BC_PERIODIC) return MHD_ERR_PARAM;
     if (p->precision != MHD_FP64 && p->precision != MHD_MIXED) return MHD_ERR_PARAM;

@@ -29,11 +29,14 @@ __device__ __forceinline__ ptrdiff_t ix(int n1, int n2, int i, int j, int k) {
 }
 
 // 1-based (i,j,k) of flat index t over extents (a, b, *)
+// flat index -> 1-based (i, j, k) of an (a, b, *) box; 32-bit unsigned
+// arithmetic (mhd_create keeps every index space below 2^31)
 __device__ __forceinline__ void unflat(long t, int a, int b, int& i, int& j, int& k) {
-    long r = t / a;
-    i = (int)(t - r * a) + 1;
-    long s = r / b;
-    j = (int)(r - s * b) + 1;
+    const unsigned u = (unsigned)t, ua = (unsigned)a, ub = (unsigned)b;
+    const unsigned r = u / ua;
+    i = (int)(u - r * ua) + 1;
+    const unsigned s = r / ub;
+    j = (int)(r - s * ub) + 1;
     k = (int)s + 1;
 }
 

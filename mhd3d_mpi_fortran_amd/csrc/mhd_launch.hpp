@@ -67,7 +67,7 @@ struct mhd_ctx {
     double prof_ms[MHD_K_NCLASS] = {};
     int prof_n[MHD_K_NCLASS] = {};
     SurrogateState* sur = nullptr;
-    float sur_theta = 1.0f;    // smoothness threshold of the learned reconstruction
+    float sur_theta = 0.25f;   // smoothness threshold of the learned reconstruction
     bool sur_stats = false;    // count faces / fallbacks
 };
 

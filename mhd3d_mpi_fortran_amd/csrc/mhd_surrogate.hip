@@ -46,7 +46,11 @@ __device__ __forceinline__ float act_grad(float z) {
     const float s = 1.f / (1.f + expf(-GK * z));
     return s + GK * z * s * (1.f - s);
 }
-__device__ __forceinline__ float act_fast(float z) { return z * __frcp_rn(1.f + __expf(-GK * z)); }
+// inference form: exp2 and the reciprocal straight from the hardware
+// (v_exp_f32, v_rcp_f32); the result is rounded to bf16 right after
+__device__ __forceinline__ float act_fast(float z) {
+    return z * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(z * (-GK * 1.4426950408889634f)));
+}
 
 // 5-point window of one variable in arithmetic type A: values, centre
 // c = (f(j-1)+f(j))/2 and range m = max |w - c|
@@ -107,9 +111,12 @@ struct State {
     float* scratch = nullptr;  // per-sample activations/deltas of one minibatch
     long scratch_n = 0;
     float* loss = nullptr;     // device loss accumulator
-    unsigned short* wbf = nullptr;  // bf16 copies for inference: W1t[40][32] W2t[32][32] W3t[32][32]
-    float* bias = nullptr;     // b1[32] b2[32] b3[16] (fp32)
+    unsigned short* wbf = nullptr;  // bf16 MFMA A operands per lane (k_pack_bf16)
+    float* bias = nullptr;     // biases in D-register order per half-wave (fp32)
     unsigned long long* stats = nullptr;  // faces evaluated, fallbacks
+    int* fix = nullptr;                   // fallback face lists of the three directions
+    long fix_n = 0;
+    unsigned long long* nfix = nullptr;   // their lengths
     long step = 0;
     bool have_weights = false;
 };
@@ -273,219 +280,304 @@ __global__ void k_zero_f(float* __restrict__ p, int n) {
     if (a < n) p[a] = 0.f;
 }
 
-// bf16 (round to nearest even) copies of the weights, transposed to the
-// MFMA B-operand order [k][j]; W3 padded to 32 outputs with zeros
+// ---------------------------------------------------------------------------
+// bf16 MFMA forward, 64 faces per wavefront, transposed form
+//
+// Each layer computes H^T = W . X^T + b with v_mfma_f32_32x32x16_bf16
+// (M=N=32, K=16): A = the weights (32 rows of the layer's outputs), B = the
+// layer's inputs for 32 faces (one face per column), so the 64 faces of a
+// wave are two column tiles.  Lane layouts of this instruction:
+//   A: lane l holds A[l%32][8*(l/32)+v], v=0..7
+//   B: lane l holds B[8*(l/32)+v][l%32]
+//   D: lane l, register r holds D[drow(r, l/32)][l%32],
+//      drow(r, h) = 8*(r/4) + 4*h + r%4
+// The D registers r = 8kb..8kb+7 of a lane are exactly the B operand of
+// k-block kb of the next layer once the next layer's weight columns are
+// permuted to k = 16kb + 8h + v <-> hidden unit drow(8kb + v, h); that
+// permutation is applied when the weights are packed, so hidden
+// activations never leave the registers.  Inputs and outputs of a face
+// live in one lane (lane l owns face l of the wave); the two halves of the
+// wave swap the parts the other half's tile needs (one shuffle per packed
+// pair).  The packed weights sit in LDS (one copy per block, read as the
+// A operands of each layer); activations never leave the registers.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+
 __device__ __forceinline__ unsigned short to_bf16(float f) {
     unsigned int u = __float_as_uint(f);
     u += 0x7fffu + ((u >> 16) & 1u);
     return (unsigned short)(u >> 16);
 }
+// two floats to packed bf16, round to nearest even (v_cvt_pk_bf16_f32)
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned int pack2(float a, float b) {
+    bf2 v;
+    v.x = (__bf16)a;
+    v.y = (__bf16)b;
+    return __builtin_bit_cast(unsigned int, v);
+}
+__host__ __device__ constexpr int drow(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
+
+constexpr int NSLOT = 7;        // A operands per lane: W1 k-blocks 0-2, W2 0-1, W3 0-1
+constexpr int NBIAS = 40;       // per half-wave: b1 rows drow(0..15), b2 rows drow(0..15), b3 rows drow(0..7)
+constexpr int WBF_N = 64 * NSLOT * 8;
+
+// pack: wbf[(lane*7 + slot)*8 + v] (bf16), bias[h*40 + q] (fp32)
 __global__ void k_pack_bf16(const float* __restrict__ P, unsigned short* __restrict__ wbf, float* __restrict__ bias) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a < NI * NH) {  // W1t[k][j]
-        const int k = a / NH, j = a % NH;
-        wbf[a] = to_bf16(P[OFF_W1 + j * NI + k]);
+    if (a < WBF_N) {
+        const int v = a & 7, slot = (a >> 3) % NSLOT, l = (a >> 3) / NSLOT;
+        const int j = l & 31, h = l >> 5;
+        float w = 0.f;
+        if (slot < 3) {
+            const int k = 16 * slot + 8 * h + v;
+            if (k < NI) w = P[OFF_W1 + j * NI + k];
+        } else if (slot < 5) {
+            w = P[OFF_W2 + j * NH + drow(8 * (slot - 3) + v, h)];
+        } else if (j < NO) {
+            w = P[OFF_W3 + j * NH + drow(8 * (slot - 5) + v, h)];
+        }
+        wbf[a] = to_bf16(w);
     }
-    if (a < NH * NH) {  // W2t[k][j]
-        const int k = a / NH, j = a % NH;
-        wbf[NI * NH + a] = to_bf16(P[OFF_W2 + j * NH + k]);
-        wbf[NI * NH + NH * NH + a] = j < NO ? to_bf16(P[OFF_W3 + j * NH + k]) : (unsigned short)0;  // W3t[k][j]
-    }
-    if (a < NH) {
-        bias[a] = P[OFF_B1 + a];
-        bias[NH + a] = P[OFF_B2 + a];
-        if (a < NO) bias[2 * NH + a] = P[OFF_B3 + a];
+    if (a < 2 * NBIAS) {
+        const int h = a / NBIAS, q = a % NBIAS;
+        bias[a] = q < 16 ? P[OFF_B1 + drow(q, h)] : (q < 32 ? P[OFF_B2 + drow(q - 16, h)] : P[OFF_B3 + drow(q - 32, h)]);
     }
 }
 
-// ---------------------------------------------------------------------------
-// bf16 MFMA forward of 64 faces per wavefront
-// v_mfma_f32_32x32x8bf16_1k lane layouts (M=N=32, K=8):
-//   A: lane l holds A[l%32][4*(l/32)+v], v=0..3
-//   B: lane l holds B[4*(l/32)+v][l%32]
-//   D: lane l, register r holds D[8*(r/4)+4*(l/32)+(r%4)][l%32]
-// Each wave stages its 64x40 bf16 inputs, hidden activations and outputs
-// through its own LDS slice (no cross-wave sharing, so only the wave's own
-// s_barrier-free ordering is needed; __syncthreads is used between phases).
-// ---------------------------------------------------------------------------
-typedef short s4 __attribute__((ext_vector_type(4)));
-typedef float f16v __attribute__((ext_vector_type(16)));
-
-struct WaveLds {
-    unsigned short xa[64][NI + 8];  // inputs (bf16), padded row
-    unsigned short h[64][NH + 8];   // hidden activations (bf16)
-    float y[64][NO + 1];            // outputs
+// the packed net of one block, staged in LDS once and read per layer
+struct NetW {
+    bf8 a[64][NSLOT];
+    float b[2][NBIAS];
 };
 
-__device__ __forceinline__ s4 ld_a(const unsigned short* row, int k0) {
-    s4 r;
-    r.x = (short)row[k0];
-    r.y = (short)row[k0 + 1];
-    r.z = (short)row[k0 + 2];
-    r.w = (short)row[k0 + 3];
-    return r;
+__device__ __forceinline__ void load_net(NetW& W, const unsigned short* __restrict__ wbf,
+                                         const float* __restrict__ bias) {
+    const bf8* src = reinterpret_cast<const bf8*>(wbf);
+    bf8* dst = &W.a[0][0];
+    for (int q = threadIdx.x; q < 64 * NSLOT; q += blockDim.x) dst[q] = src[q];
+    for (int q = threadIdx.x; q < 2 * NBIAS; q += blockDim.x) (&W.b[0][0])[q] = bias[q];
+    __syncthreads();
 }
 
-// B operands of one layer for this lane: nk k-blocks of 8
-template <int NK>
-__device__ __forceinline__ void ld_b(const unsigned short* __restrict__ wt, int lane, s4 (&b)[NK]) {
-    const int j = lane & 31, kk = 4 * (lane >> 5);
+__device__ __forceinline__ bf8 as_bf8(u4 u) { return __builtin_bit_cast(bf8, u); }
+
+__device__ __forceinline__ unsigned int swap_halves(unsigned int u) { return (unsigned int)__shfl_xor((int)u, 32, 64); }
+__device__ __forceinline__ float swap_halves(float f) { return __shfl_xor(f, 32, 64); }
+
+// hidden layer epilogue: activation of D, packed as the next layer's B
+__device__ __forceinline__ void hidden_b(const f16v& d, bf8 (&b)[2]) {
 #pragma unroll
-    for (int kb = 0; kb < NK; ++kb) {
-        s4 r;
-        r.x = (short)wt[(kb * 8 + kk + 0) * NH + j];
-        r.y = (short)wt[(kb * 8 + kk + 1) * NH + j];
-        r.z = (short)wt[(kb * 8 + kk + 2) * NH + j];
-        r.w = (short)wt[(kb * 8 + kk + 3) * NH + j];
-        b[kb] = r;
+    for (int kb = 0; kb < 2; ++kb) {
+        u4 u;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) u[p] = pack2(act_fast(d[8 * kb + 2 * p]), act_fast(d[8 * kb + 2 * p + 1]));
+        b[kb] = as_bf8(u);
     }
 }
 
-// one 32-row tile: D = A(rows r0..r0+31 of src, K = 8*NK) * B
-template <int NK, int LD>
-__device__ __forceinline__ f16v mfma_tile(const unsigned short (*src)[LD], int r0, int lane, const s4 (&b)[NK]) {
-    f16v acc = {};
-    const int i = lane & 31, kk = 4 * (lane >> 5);
+// MLP outputs (without lin and gate) of this lane's face from its 40
+// normalised inputs; every lane of the wave must call it
+__device__ __forceinline__ void mlp_core(const NetW& W, const float* x, int lane, float* y) {
+    const bool hi = (lane >> 5) != 0;
+    bf8 t0[3], t1[3];
 #pragma unroll
-    for (int kb = 0; kb < NK; ++kb) {
-        const s4 a = ld_a(src[r0 + i], kb * 8 + kk);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b[kb], acc, 0, 0, 0);
-    }
-    return acc;
-}
-
-// hidden layer epilogue: bias + activation, bf16 into h (or fp32 into y)
-__device__ __forceinline__ void store_hidden(WaveLds& L, const f16v& acc, int r0, int lane,
-                                             const float* __restrict__ bias) {
-    const int j = lane & 31, ib = 4 * (lane >> 5);
-    const float bj = bias[j];
+    for (int kb = 0; kb < 3; ++kb) {
+        u4 lo4, hi4;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int i = 8 * (r >> 2) + ib + (r & 3);
-        L.h[r0 + i][j] = to_bf16(act_fast(acc[r] + bj));
-    }
-}
-
-// forward of the 64 faces whose normalised inputs sit in L.xa; y in L.y
-__device__ __forceinline__ void mlp64(WaveLds& L, int lane, const unsigned short* __restrict__ wbf,
-                                      const float* __restrict__ bias) {
-    s4 b1[5], b2[4], b3[4];
-    ld_b<5>(wbf, lane, b1);
-    ld_b<4>(wbf + NI * NH, lane, b2);
-    ld_b<4>(wbf + NI * NH + NH * NH, lane, b3);
-    __syncthreads();
-    f16v h0 = mfma_tile<5>(L.xa, 0, lane, b1);
-    f16v h1 = mfma_tile<5>(L.xa, 32, lane, b1);
-    store_hidden(L, h0, 0, lane, bias);
-    store_hidden(L, h1, 32, lane, bias);
-    __syncthreads();
-    h0 = mfma_tile<4>(L.h, 0, lane, b2);
-    h1 = mfma_tile<4>(L.h, 32, lane, b2);
-    __syncthreads();
-    store_hidden(L, h0, 0, lane, bias + NH);
-    store_hidden(L, h1, 32, lane, bias + NH);
-    __syncthreads();
-    h0 = mfma_tile<4>(L.h, 0, lane, b3);
-    h1 = mfma_tile<4>(L.h, 32, lane, b3);
-    const int j = lane & 31, ib = 4 * (lane >> 5);
-    if (j < NO) {
-        const float bj = bias[2 * NH + j];
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int i = 8 * (r >> 2) + ib + (r & 3);
-            L.y[i][j] = h0[r] + bj;
-            L.y[32 + i][j] = h1[r] + bj;
+        for (int p = 0; p < 4; ++p) {
+            const int k0 = 16 * kb + 2 * p, k1 = k0 + 8;
+            lo4[p] = pack2(k0 < NI ? x[k0] : 0.f, k0 + 1 < NI ? x[k0 + 1] : 0.f);
+            hi4[p] = pack2(k1 < NI ? x[k1] : 0.f, k1 + 1 < NI ? x[k1 + 1] : 0.f);
         }
+        u4 keep, recv;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            keep[p] = hi ? hi4[p] : lo4[p];
+            recv[p] = swap_halves(hi ? lo4[p] : hi4[p]);
+        }
+        t0[kb] = as_bf8(hi ? recv : keep);
+        t1[kb] = as_bf8(hi ? keep : recv);
     }
-    __syncthreads();
+    const bf8* A = W.a[lane];
+    const float* B = W.b[lane >> 5];
+    f16v d0, d1;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) d0[r] = d1[r] = B[r];
+#pragma unroll
+    for (int kb = 0; kb < 3; ++kb) {
+        const bf8 a = A[kb];
+        d0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, t0[kb], d0, 0, 0, 0);
+        d1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, t1[kb], d1, 0, 0, 0);
+    }
+    bf8 h0[2], h1[2];
+    hidden_b(d0, h0);
+    hidden_b(d1, h1);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) d0[r] = d1[r] = B[16 + r];
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        const bf8 a = A[3 + kb];
+        d0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, h0[kb], d0, 0, 0, 0);
+        d1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, h1[kb], d1, 0, 0, 0);
+    }
+    hidden_b(d0, h0);
+    hidden_b(d1, h1);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) d0[r] = d1[r] = r < 8 ? B[32 + r] : 0.f;
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        const bf8 a = A[5 + kb];
+        d0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, h0[kb], d0, 0, 0, 0);
+        d1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, h1[kb], d1, 0, 0, 0);
+    }
+    // lane l owns face l: tile 0 for the low half, tile 1 for the high half;
+    // output rows drow(r, h), r < 8, come from the half h
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const float own = hi ? d1[r] : d0[r];
+        const float other = swap_halves(hi ? d0[r] : d1[r]);
+        y[drow(r, 0)] = hi ? other : own;
+        y[drow(r, 1)] = hi ? own : other;
+    }
 }
 
 // forward of host-given normalised inputs (tests): n rows of 40 floats
 __global__ void __launch_bounds__(kBlock) k_infer(const float* __restrict__ x, long n,
                                                  const unsigned short* __restrict__ wbf,
                                                  const float* __restrict__ bias, float* __restrict__ y) {
-    __shared__ WaveLds lds[kBlock / 64];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    WaveLds& L = lds[wv];
+    const int lane = threadIdx.x & 63;
+    __shared__ NetW W;
+    load_net(W, wbf, bias);
     const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
-#pragma unroll
-    for (int a = 0; a < NI; ++a) L.xa[lane][a] = to_bf16(row < n ? x[row * NI + a] : 0.f);
-    float xr[NI];
+    float xr[NI], out[NO];
 #pragma unroll
     for (int a = 0; a < NI; ++a) xr[a] = row < n ? x[row * NI + a] : 0.f;
-    mlp64(L, lane, wbf, bias);
+    mlp_core(W, xr, lane, out);
     if (row < n)
         for (int o = 0; o < NO; ++o)
-            y[row * NO + o] = lin4(xr + 5 * (o & 7)) + rough5(xr + 5 * (o & 7)) * L.y[lane][o];
+            y[row * NO + o] = lin4(xr + 5 * (o & 7)) + rough5(xr + 5 * (o & 7)) * out[o];
 }
 
 // ---------------------------------------------------------------------------
-// fp32 face flux with the learned reconstruction (PDM fallback)
+// fp32 face flux with the learned reconstruction; each block covers
+// SUR_GROUPS x 256 consecutive faces, so the weights staged in LDS serve
+// several groups.  Faces that fail the smoothness or positivity check are
+// appended to a list (one atomic per wavefront) and get the PDM flux from
+// k_flux_fix, which keeps PDM's registers out of this kernel.
 // ---------------------------------------------------------------------------
+constexpr int SUR_GROUPS = 4;
+
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_flux_sur(Dims d, Coef<float> c, Prim<float> q, float theta,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_flux_sur(Dims d, Coef<float> c, Prim<float> q, float theta,
                                                     const unsigned short* __restrict__ wbf,
                                                     const float* __restrict__ bias, float* __restrict__ F, long nface,
+                                                    int* __restrict__ fix, unsigned long long* __restrict__ nfix,
                                                     unsigned long long* __restrict__ stats) {
-    __shared__ WaveLds lds[kBlock / 64];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    WaveLds& L = lds[wv];
-    const long t = gtid();
-    const bool live = t < nface;
-    const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
-    int i = 1, j = 1, k = 1;
-    if (live) unflat(t, ii, jj, i, j, k);
-    const ptrdiff_t base = ix(d.nx, d.ny, d.ng + i, d.ng + j, d.ng + k);
+    const int lane = threadIdx.x & 63;
+    __shared__ NetW W;
+    load_net(W, wbf, bias);
     const ptrdiff_t st = D == 1 ? 1 : (D == 2 ? (ptrdiff_t)d.nx : (ptrdiff_t)d.nx * d.ny);
     const float* f[8] = {q.rho, q.vx, q.vy, q.vz, q.p, q.bx, q.by, q.bz};
-    float x[NI], cc[8], mm[8], sc[8], ss[8], lin[8], gate[8], rough = 0.f;
+    const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
+    for (int grp = 0; grp < SUR_GROUPS; ++grp) {
+        const long t = ((long)blockIdx.x * SUR_GROUPS + grp) * blockDim.x + threadIdx.x;
+        const bool live = t < nface;
+        if (__ballot(live) == 0ull) break;  // wave-uniform: later groups are past the end too
+        int i = 1, j = 1, k = 1;
+        if (live) unflat(t, ii, jj, i, j, k);
+        const ptrdiff_t base = ix(d.nx, d.ny, d.ng + i, d.ng + j, d.ng + k);
+        float x[NI], cc[8], mm[8], sc[8], rough = 0.f;
 #pragma unroll
-    for (int v = 0; v < 8; ++v) {
-        if (live) load5<float>(f[v], base, st, x + 5 * v, cc[v], mm[v]);
-        else {
+        for (int v = 0; v < 8; ++v) {
+            if (live) load5<float>(f[v], base, st, x + 5 * v, cc[v], mm[v]);
+            else {
 #pragma unroll
-            for (int a = 0; a < 5; ++a) x[5 * v + a] = 0.f;
-            cc[v] = mm[v] = 0.f;
+                for (int a = 0; a < 5; ++a) x[5 * v + a] = 0.f;
+                cc[v] = mm[v] = 0.f;
+            }
         }
-    }
-    var_scales<float>(cc, sc);
+        var_scales<float>(cc, sc);
+        // per variable, in physical units: the linear face value A, the gate
+        // times the window scale G, and the clamp range [lo, hi] of the two
+        // cells next to the face (PDM's own bound, mhd3d3.f90:1142-1166)
+        float A[8], G[8], lo[8], hi[8];
 #pragma unroll
-    for (int v = 0; v < 8; ++v) {
-        ss[v] = norm_scale<float>(mm[v], sc[v]);
-        const float r = __frcp_rn(ss[v]);
+        for (int v = 0; v < 8; ++v) {
+            const float ss = norm_scale<float>(mm[v], sc[v]);
+            const float w1 = x[5 * v + 1], w2 = x[5 * v + 2];
+            lo[v] = fminf(w1, w2);
+            hi[v] = fmaxf(w1, w2);
+            const float r = __frcp_rn(ss);
 #pragma unroll
-        for (int a = 0; a < 5; ++a) x[5 * v + a] = (x[5 * v + a] - cc[v]) * r;
-        gate[v] = rough5(x + 5 * v);
-        rough = fmaxf(rough, gate[v]);
-        lin[v] = lin4(x + 5 * v);
-#pragma unroll
-        for (int a = 0; a < 5; ++a) L.xa[lane][5 * v + a] = to_bf16(x[5 * v + a]);
-    }
-    mlp64(L, lane, wbf, bias);
-    // the face states, clamped to the range of the two cells next to the
-    // face (PDM's own bound, mhd3d3.f90:1142-1166)
-    float lv[8], rv[8];
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-        const float lo = fminf(x[5 * v + 1], x[5 * v + 2]), hi = fmaxf(x[5 * v + 1], x[5 * v + 2]);
-        lv[v] = cc[v] + ss[v] * fminf(hi, fmaxf(lo, lin[v] + gate[v] * L.y[lane][v]));
-        rv[v] = cc[v] + ss[v] * fminf(hi, fmaxf(lo, lin[v] + gate[v] * L.y[lane][8 + v]));
-    }
-    bool fb = !(rough <= theta) || !(lv[0] > 0.f) || !(rv[0] > 0.f) || !(lv[4] > 0.f) || !(rv[4] > 0.f);
-    if (live && fb) {
-#pragma unroll
-        for (int v = 0; v < 8; ++v) recon_line<float, PDM>(f[v], base, st, c, lv[v], rv[v]);
-    }
-    if (stats) {
-        unsigned long long nl = __ballot(live), nf = __ballot(live && fb);
-        if (lane == 0) {
-            atomicAdd(&stats[0], (unsigned long long)__popcll(nl));
-            atomicAdd(&stats[1], (unsigned long long)__popcll(nf));
+            for (int a = 0; a < 5; ++a) x[5 * v + a] = (x[5 * v + a] - cc[v]) * r;
+            const float gate = rough5(x + 5 * v);
+            rough = fmaxf(rough, gate);
+            A[v] = cc[v] + ss * lin4(x + 5 * v);
+            G[v] = ss * gate;
         }
+        float y[NO];
+        mlp_core(W, x, lane, y);
+        float lv[8], rv[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            lv[v] = fminf(hi[v], fmaxf(lo[v], A[v] + G[v] * y[v]));
+            rv[v] = fminf(hi[v], fmaxf(lo[v], A[v] + G[v] * y[8 + v]));
+        }
+        const bool fb = !(rough <= theta) || !(lv[0] > 0.f) || !(rv[0] > 0.f) || !(lv[4] > 0.f) || !(rv[4] > 0.f);
+        // fallback faces go to a list that k_flux_fix evaluates with PDM
+        const unsigned long long fm = __ballot(live && fb);
+        if (fm) {
+            unsigned long long first = 0;
+            const int leader = __ffsll((long long)fm) - 1;
+            if (lane == leader) first = atomicAdd(nfix, (unsigned long long)__popcll(fm));
+            first = __shfl(first, leader, 64);
+            if (live && fb) fix[first + __popcll(fm & ((1ull << lane) - 1ull))] = (int)t;
+        }
+        if (stats) {
+            const unsigned long long nl = __ballot(live);
+            if (lane == 0) {
+                atomicAdd(&stats[0], (unsigned long long)__popcll(nl));
+                atomicAdd(&stats[1], (unsigned long long)__popcll(fm));
+            }
+        }
+        if (live && !fb) face_flux<float, D>(c, lv, rv, F, t, nface);
     }
-    if (!live) return;
-    face_flux<float, D>(c, lv, rv, F, t, nface);
 }
+
+// PDM flux of the listed fallback faces (fp32, as the plain mixed path)
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_flux_fix(Dims d, Coef<float> c, Prim<float> q,
+                                                    const int* __restrict__ fix,
+                                                    const unsigned long long* __restrict__ nfix,
+                                                    float* __restrict__ F, long nface) {
+    const long n = (long)*nfix;
+    const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
+    const ptrdiff_t st = D == 1 ? 1 : (D == 2 ? (ptrdiff_t)d.nx : (ptrdiff_t)d.nx * d.ny);
+    for (long e = gtid(); e < n; e += (long)gridDim.x * blockDim.x) {
+        const long t = fix[e];
+        int i, j, k;
+        unflat(t, ii, jj, i, j, k);
+        const ptrdiff_t base = ix(d.nx, d.ny, d.ng + i, d.ng + j, d.ng + k);
+        float l[8], r[8];
+        recon_line<float, PDM>(q.rho, base, st, c, l[0], r[0]);
+        recon_line<float, PDM>(q.vx, base, st, c, l[1], r[1]);
+        recon_line<float, PDM>(q.vy, base, st, c, l[2], r[2]);
+        recon_line<float, PDM>(q.vz, base, st, c, l[3], r[3]);
+        recon_line<float, PDM>(q.p, base, st, c, l[4], r[4]);
+        recon_line<float, PDM>(q.bx, base, st, c, l[5], r[5]);
+        recon_line<float, PDM>(q.by, base, st, c, l[6], r[6]);
+        recon_line<float, PDM>(q.bz, base, st, c, l[7], r[7]);
+        face_flux<float, D>(c, l, r, F, t, nface);
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long* __restrict__ p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
 
 }  // namespace sur
 
@@ -501,8 +593,8 @@ static SurrogateState* sstate(mhd_ctx* c) {
         bool ok = hipMalloc(&s->p, pb) == hipSuccess && hipMalloc(&s->g, pb) == hipSuccess &&
                   hipMalloc(&s->m, pb) == hipSuccess && hipMalloc(&s->v, pb) == hipSuccess &&
                   hipMalloc(&s->count, 64) == hipSuccess && hipMalloc(&s->loss, 64) == hipSuccess &&
-                  hipMalloc(&s->wbf, (sur::NI * sur::NH + 2 * sur::NH * sur::NH) * 2) == hipSuccess &&
-                  hipMalloc(&s->bias, (2 * sur::NH + sur::NO) * sizeof(float)) == hipSuccess &&
+                  hipMalloc(&s->wbf, sur::WBF_N * sizeof(unsigned short)) == hipSuccess &&
+                  hipMalloc(&s->bias, 2 * sur::NBIAS * sizeof(float)) == hipSuccess &&
                   hipMalloc(&s->stats, 64) == hipSuccess;
         if (!ok) {
             delete s;
@@ -522,7 +614,8 @@ static SurrogateState* sstate(mhd_ctx* c) {
 void surrogate_free(mhd_ctx* c) {
     SurrogateState* s = c->sur;
     if (!s) return;
-    void* ps[] = {s->p, s->g, s->m, s->v, s->samples, s->count, s->scratch, s->loss, s->wbf, s->bias, s->stats};
+    void* ps[] = {s->p, s->g, s->m, s->v, s->samples, s->count, s->scratch, s->loss, s->wbf, s->bias, s->stats,
+                  s->fix, s->nfix};
     for (void* p : ps)
         if (p) hipFree(p);
     delete s;
@@ -535,16 +628,42 @@ void launch_flux_surrogate(mhd_ctx* c, Src<float> s) {
         launch_flux_fp32(c, s);
         return;
     }
+    const long nfx = c->nf[0] + c->nf[1] + c->nf[2];
+    if (S->fix_n < nfx) {
+        if (S->fix) hipFree(S->fix);
+        if (!S->nfix && hipMalloc(&S->nfix, 64) != hipSuccess) S->nfix = nullptr;
+        if (hipMalloc(&S->fix, (size_t)nfx * sizeof(int)) != hipSuccess) S->fix = nullptr;
+        S->fix_n = S->fix && S->nfix ? nfx : 0;
+        if (!S->fix_n) {  // no room for the fallback lists: plain fp32 PDM
+            launch_flux_fp32(c, s);
+            return;
+        }
+    }
     const Coef<float> co = make_coef<float>(c);
     const Prim<float> q{s.rho, s.vx, s.vy, s.vz, s.p, s.bx, s.by, s.bz};
     const float theta = c->sur_theta;
     unsigned long long* st = c->sur_stats ? S->stats : nullptr;
-    launch(c, MHD_K_SURR, blocks(c->nf[0]), dim3(kBlock), sur::k_flux_sur<1>, c->d, co, q, theta,
-           (const unsigned short*)S->wbf, (const float*)S->bias, (float*)c->flux[0], c->nf[0], st);
-    launch(c, MHD_K_SURR, blocks(c->nf[1]), dim3(kBlock), sur::k_flux_sur<2>, c->d, co, q, theta,
-           (const unsigned short*)S->wbf, (const float*)S->bias, (float*)c->flux[1], c->nf[1], st);
-    launch(c, MHD_K_SURR, blocks(c->nf[2]), dim3(kBlock), sur::k_flux_sur<3>, c->d, co, q, theta,
-           (const unsigned short*)S->wbf, (const float*)S->bias, (float*)c->flux[2], c->nf[2], st);
+    launch(c, MHD_K_FLUX, dim3(1), dim3(64), sur::k_zero_u64, S->nfix, 3);
+    int* fx[3] = {S->fix, S->fix + c->nf[0], S->fix + c->nf[0] + c->nf[1]};
+    const unsigned gsur[3] = {(unsigned)((c->nf[0] + sur::SUR_GROUPS * kBlock - 1) / (sur::SUR_GROUPS * kBlock)),
+                              (unsigned)((c->nf[1] + sur::SUR_GROUPS * kBlock - 1) / (sur::SUR_GROUPS * kBlock)),
+                              (unsigned)((c->nf[2] + sur::SUR_GROUPS * kBlock - 1) / (sur::SUR_GROUPS * kBlock))};
+    const unsigned short* wbf = S->wbf;
+    const float* bias = S->bias;
+    launch(c, MHD_K_FLUX, dim3(gsur[0]), dim3(kBlock), sur::k_flux_sur<1>, c->d, co, q, theta, wbf, bias,
+           (float*)c->flux[0], c->nf[0], fx[0], S->nfix + 0, st);
+    launch(c, MHD_K_FLUX, dim3(gsur[1]), dim3(kBlock), sur::k_flux_sur<2>, c->d, co, q, theta, wbf, bias,
+           (float*)c->flux[1], c->nf[1], fx[1], S->nfix + 1, st);
+    launch(c, MHD_K_FLUX, dim3(gsur[2]), dim3(kBlock), sur::k_flux_sur<3>, c->d, co, q, theta, wbf, bias,
+           (float*)c->flux[2], c->nf[2], fx[2], S->nfix + 2, st);
+    // the fallback lists: a fixed grid walks whatever length each list has
+    const dim3 gfix(1024);
+    launch(c, MHD_K_FLUX, gfix, dim3(kBlock), sur::k_flux_fix<1>, c->d, co, q, (const int*)fx[0],
+           (const unsigned long long*)(S->nfix + 0), (float*)c->flux[0], c->nf[0]);
+    launch(c, MHD_K_FLUX, gfix, dim3(kBlock), sur::k_flux_fix<2>, c->d, co, q, (const int*)fx[1],
+           (const unsigned long long*)(S->nfix + 1), (float*)c->flux[1], c->nf[1]);
+    launch(c, MHD_K_FLUX, gfix, dim3(kBlock), sur::k_flux_fix<3>, c->d, co, q, (const int*)fx[2],
+           (const unsigned long long*)(S->nfix + 2), (float*)c->flux[2], c->nf[2]);
 }
 
 static int sync_or(mhd_ctx* c) {
@@ -561,7 +680,7 @@ int mhd_surrogate_set_weights(mhd_ctx* c, const float* w, int n) {
     SurrogateState* S = sstate(c);
     if (!S) return MHD_ERR_HIP;
     if (hipMemcpyAsync(S->p, w, n * sizeof(float), hipMemcpyHostToDevice, c->s) != hipSuccess) return MHD_ERR_HIP;
-    launch(c, MHD_K_SURR, dim3((sur::NI * sur::NH + 255) / 256), dim3(256), sur::k_pack_bf16, (const float*)S->p,
+    launch(c, MHD_K_SURR, dim3((sur::WBF_N + 255) / 256), dim3(256), sur::k_pack_bf16, (const float*)S->p,
            S->wbf, S->bias);
     S->have_weights = true;
     return sync_or(c);
@@ -737,7 +856,7 @@ int mhd_surrogate_train(mhd_ctx* c, int epochs, int batch, double lr, unsigned s
         if (hipStreamSynchronize(c->s) != hipSuccess) return MHD_ERR_HIP;
         tot = (double)h / (double)(nb * batch * sur::NO);
     }
-    launch(c, MHD_K_SURR, dim3((sur::NI * sur::NH + 255) / 256), dim3(256), sur::k_pack_bf16, (const float*)S->p,
+    launch(c, MHD_K_SURR, dim3((sur::WBF_N + 255) / 256), dim3(256), sur::k_pack_bf16, (const float*)S->p,
            S->wbf, S->bias);
     if (mean_loss) *mean_loss = tot;
     return sync_or(c);

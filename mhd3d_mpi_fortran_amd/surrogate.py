@@ -16,7 +16,7 @@ from . import gpu, ics
 
 NPARAM = 2896
 NSAMPLE = 56
-THETA = 0.5          # smoothness threshold used for training and inference
+THETA = 0.25         # smoothness threshold used for training and inference
 
 
 def _pf(a):
@@ -116,9 +116,9 @@ def build_training_set(n=128, capacity=1 << 24, theta=THETA, stride=4, snapshots
     return s, total
 
 
-def train_surrogate(n=128, capacity=1 << 24, epochs=2, batch=1024, lr=5e-3, seed=0, log=None):
+def train_surrogate(n=128, capacity=1 << 24, epochs=2, batch=1024, lr=5e-3, seed=0, stride=8, log=None):
     """collect samples and train; returns (weights, final mean loss, samples)."""
-    s, total = build_training_set(n=n, capacity=capacity, log=log)
+    s, total = build_training_set(n=n, capacity=capacity, stride=stride, log=log)
     loss = train(s, epochs=epochs, batch=batch, lr=lr, seed=seed)
     w = get_weights(s)
     s.close()
