@@ -70,6 +70,16 @@ const char *mhd_error_string(int code);
  * (init_geom, mhd3d3.f90:3483-3485; tensor-product grid) */
 int mhd_set_geometry(mhd_ctx *ctx, const double *dx, const double *dy, const double *dz);
 
+/* affine metric (no counterpart in the reference, whose grid is the tensor
+ * product of init_geom, mhd3d3.f90:3483-3485): physical x = A x_comp with
+ * A (row-major 3x3, det A > 0) applied to the tensor-product grid of
+ * mhd_set_geometry -- e.g. a sheared box.  Fluxes are taken along the face
+ * normals, constrained transport and the CFL use the faces' areas and the
+ * cells' volume; vx,vy,vz and the cell B are physical Cartesian components,
+ * bi,bj,bk the normal components B.n of the three face families.  Calling it
+ * switches the context to the metric kernels (also for A = I). */
+int mhd_set_metric(mhd_ctx *ctx, const double *A);
+
 /* copy the primitive state in / out (full ghost-inclusive arrays).  Any of
  * the download pointers may be NULL. */
 int mhd_upload_state(mhd_ctx *ctx, const double *rho, const double *vx, const double *vy, const double *vz,

@@ -174,21 +174,108 @@ void mhd_destroy(mhd_ctx* c) {
         hipEventDestroy(r.b);
     }
     surrogate_free(c);
+    if (c->metbuf) hipFree(c->metbuf);
     if (c->arena) hipFree(c->arena);
     if (c->s) hipStreamDestroy(c->s);
     delete c;
 }
 
+static int upload_metric_spacing(mhd_ctx* c);
+
 int mhd_set_geometry(mhd_ctx* c, const double* dx, const double* dy, const double* dz) {
     if (!c || !dx || !dy || !dz) return MHD_ERR_PARAM;
     hipSetDevice(c->p.device);
+    c->hdx.assign(dx, dx + c->d.nx);
+    c->hdy.assign(dy, dy + c->d.ny);
+    c->hdz.assign(dz, dz + c->d.nz);
     if (hipMemcpyAsync(c->dx, dx, c->d.nx * sizeof(double), hipMemcpyHostToDevice, c->s) != hipSuccess ||
         hipMemcpyAsync(c->dy, dy, c->d.ny * sizeof(double), hipMemcpyHostToDevice, c->s) != hipSuccess ||
         hipMemcpyAsync(c->dz, dz, c->d.nz * sizeof(double), hipMemcpyHostToDevice, c->s) != hipSuccess)
         return MHD_ERR_HIP;
     c->have_geometry = true;
+    if (c->met.on && upload_metric_spacing(c) != MHD_OK) return MHD_ERR_HIP;
     return hipStreamSynchronize(c->s) == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
+
+// ---------------------------------------------------------------------------
+// affine metric (no counterpart in the reference): x_phys = A x_comp
+// ---------------------------------------------------------------------------
+static int upload_metric_spacing(mhd_ctx* c) {
+    if (!c->have_geometry || !c->dxm) return MHD_OK;
+    std::vector<double> h;
+    const std::vector<double>* src[3] = {&c->hdx, &c->hdy, &c->hdz};
+    double* dst[3] = {c->dxm, c->dym, c->dzm};
+    for (int a = 0; a < 3; ++a) {
+        h.resize(src[a]->size());
+        for (size_t q = 0; q < h.size(); ++q) h[q] = (*src[a])[q] / c->met.g[a];
+        if (hipMemcpy(dst[a], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            return MHD_ERR_HIP;
+    }
+    return MHD_OK;
+}
+
+int mhd_set_metric(mhd_ctx* c, const double* A) {
+    if (!c || !A) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    // columns a_d = A e_d; cofactor columns C e_d = a_(d+1) x a_(d+2)
+    double a[3][3], C[3][3];
+    for (int d = 0; d < 3; ++d)
+        for (int r = 0; r < 3; ++r) a[d][r] = A[3 * r + d];
+    for (int d = 0; d < 3; ++d) {
+        const double* p = a[(d + 1) % 3];
+        const double* q = a[(d + 2) % 3];
+        C[d][0] = p[1] * q[2] - p[2] * q[1];
+        C[d][1] = p[2] * q[0] - p[0] * q[2];
+        C[d][2] = p[0] * q[1] - p[1] * q[0];
+    }
+    const double J = a[0][0] * C[0][0] + a[0][1] * C[0][1] + a[0][2] * C[0][2];
+    if (!(J > 0.) || !std::isfinite(J)) return MHD_ERR_PARAM;
+    Metric m;
+    m.on = true;
+    for (int q = 0; q < 9; ++q) m.A[q] = A[q];
+    double len[3];
+    for (int d = 0; d < 3; ++d) {
+        len[d] = std::sqrt(C[d][0] * C[d][0] + C[d][1] * C[d][1] + C[d][2] * C[d][2]);
+        for (int r = 0; r < 3; ++r) m.n[d][r] = C[d][r] / len[d];
+        m.g[d] = len[d] / J;
+        m.ct[d] = 1. / len[d];
+    }
+    for (int d = 0; d < 3; ++d) m.ef[d] = len[(d + 1) % 3] * len[(d + 2) % 3] / J;
+    for (int r = 0; r < 3; ++r)
+        for (int d = 0; d < 3; ++d) m.M[3 * r + d] = A[3 * r + d] * m.g[d];
+    if (!c->metbuf) {
+        const size_t cb = c->ncell * sizeof(double), fb = c->ncell * sizeof(float);
+        const size_t sb = (c->d.nx + c->d.ny + c->d.nz) * sizeof(double);
+        const bool ab = c->p.stepop == 1, mx = c->p.precision == MHD_MIXED;
+        const size_t total = sb + 3 * cb + (ab && !mx ? 3 * cb : 0) + (mx ? 3 * fb : 0) + 1024;
+        if (hipMalloc(&c->metbuf, total) != hipSuccess) {
+            c->metbuf = nullptr;
+            return MHD_ERR_HIP;
+        }
+        char* q = c->metbuf;
+        auto take = [&](size_t n) {
+            char* r = q;
+            q += (n + 255) & ~size_t(255);
+            return r;
+        };
+        c->dxm = (double*)take(c->d.nx * sizeof(double));
+        c->dym = (double*)take(c->d.ny * sizeof(double));
+        c->dzm = (double*)take(c->d.nz * sizeof(double));
+        for (int d = 0; d < 3; ++d) c->w64[d] = (double*)take(cb);
+        if (ab && !mx)
+            for (int d = 0; d < 3; ++d) c->wh64[d] = (double*)take(cb);
+        if (mx)
+            for (int d = 0; d < 3; ++d) c->w32[d] = (float*)take(fb);
+    }
+    c->met = m;
+    c->dt_fresh = false;
+    if (upload_metric_spacing(c) != MHD_OK) return MHD_ERR_HIP;
+    return MHD_OK;
+}
+
+static const double* sdx(const mhd_ctx* c) { return c->met.on ? c->dxm : c->dx; }
+static const double* sdy(const mhd_ctx* c) { return c->met.on ? c->dym : c->dy; }
+static const double* sdz(const mhd_ctx* c) { return c->met.on ? c->dzm : c->dz; }
 
 int mhd_upload_state(mhd_ctx* c, const double* rho, const double* vx, const double* vy, const double* vz,
                      const double* p, const double* bi, const double* bj, const double* bk) {
@@ -227,15 +314,34 @@ static Prim<double> cur_state(mhd_ctx* c) {
     return Prim<double>{c->rho, c->vx, c->vy, c->vz, c->pr, c->bx, c->by, c->bz};
 }
 static Src<double> cur_src(mhd_ctx* c) {
-    return Src<double>{c->rho, c->vx, c->vy, c->vz, c->pr, c->bx, c->by, c->bz, c->vf, c->bi, c->bj, c->bk};
+    Src<double> s{c->rho, c->vx, c->vy, c->vz, c->pr, c->bx, c->by, c->bz, c->vf, c->bi, c->bj, c->bk, {}};
+    for (int d = 0; d < 3; ++d) s.w[d] = c->w64[d];
+    return s;
 }
 static Src<double> h_src(mhd_ctx* c) {
-    return Src<double>{c->xh[0], c->xh[1], c->xh[2], c->xh[3], c->xh[4], c->xh[5],
-                       c->xh[6], c->xh[7], c->vfh,   c->xh[8], c->xh[9], c->xh[10]};
+    Src<double> s{c->xh[0], c->xh[1], c->xh[2], c->xh[3], c->xh[4], c->xh[5],
+                  c->xh[6], c->xh[7], c->vfh,   c->xh[8], c->xh[9], c->xh[10], {}};
+    for (int d = 0; d < 3; ++d) s.w[d] = c->wh64[d];
+    return s;
 }
 static Src<float> sh_src(mhd_ctx* c) {
     float* const* h = c->sh;
-    return Src<float>{h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7], h[8], h[9], h[10], h[11]};
+    Src<float> s{h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7], h[8], h[9], h[10], h[11], {}};
+    for (int d = 0; d < 3; ++d) s.w[d] = c->w32[d];
+    return s;
+}
+static BMet bmet(mhd_ctx* c, bool w64, bool w32) {
+    BMet m{};
+    m.on = c->met.on ? 1 : 0;
+    for (int q = 0; q < 9; ++q) {
+        m.M[q] = c->met.M[q];
+        m.n[q] = c->met.n[q / 3][q % 3];
+    }
+    for (int d = 0; d < 3; ++d) {
+        m.w[d] = w64 ? c->w64[d] : nullptr;
+        m.ws[d] = w32 ? c->w32[d] : nullptr;
+    }
+    return m;
 }
 static bool mixed(const mhd_ctx* c) { return c->p.precision == MHD_MIXED; }
 
@@ -280,10 +386,13 @@ static void do_bxyz(mhd_ctx* c, bool want_vf, bool want_shadow, bool want_dt) {
     if (want_shadow && mixed(c))
         sh = Shadow{c->sh[0], c->sh[1], c->sh[2], c->sh[3], c->sh[4],  c->sh[5],
                     c->sh[6], c->sh[7], c->sh[8], c->sh[9], c->sh[10], c->sh[11]};
+    // face-normal velocities where the EMF will read them: binary64 with vf,
+    // fp32 with the stencil copies
+    const BMet m = bmet(c, want_vf && !mixed(c), want_shadow && mixed(c));
     launch(c, MHD_K_BXYZ, blocks(c->nmax), dim3(kBlock), k_bxyz, c->d, c->p.gam, c->p.cfl,
            CFaces{c->bi, c->bj, c->bk}, CCells{c->rho, c->vx, c->vy, c->vz, c->pr}, c->bx, c->by, c->bz,
-           want_vf ? c->vf : (double*)nullptr, sh, (const double*)c->dx, (const double*)c->dy,
-           (const double*)c->dz, want_dt ? c->dtpart : (double*)nullptr, c->ncell, c->nbi, c->nbj, c->nbk);
+           want_vf ? c->vf : (double*)nullptr, sh, m, sdx(c), sdy(c), sdz(c),
+           want_dt ? c->dtpart : (double*)nullptr, c->ncell, c->nbi, c->nbj, c->nbk);
     c->dt_fresh = want_dt;
 }
 
@@ -292,7 +401,7 @@ static void do_bxyz(mhd_ctx* c, bool want_vf, bool want_shadow, bool want_dt) {
 static void do_dt(mhd_ctx* c, bool set_tau) {
     if (!c->dt_fresh) {
         launch(c, MHD_K_DT, blocks(c->ncell), dim3(kBlock), k_dt_partial, c->d, c->p.gam, c->p.cfl, cur_state(c),
-               (const double*)c->dx, (const double*)c->dy, (const double*)c->dz, c->dtpart, c->ncell);
+               sdx(c), sdy(c), sdz(c), c->dtpart, c->ncell);
         launch(c, MHD_K_DT, dim3(1), dim3(1024), k_dt_final, (const double*)c->dtpart, blocks(c->ncell).x * 1L,
                c->scal + SC_DTCFL, set_tau ? c->scal + SC_TAU : (double*)nullptr);
         return;
@@ -316,15 +425,15 @@ static void do_update_t(mhd_ctx* c, double* const* out) {
     PrevRK pv{c->rhop, c->mxp, c->myp, c->mzp, c->engp};
     FluxSet<TF> F{(const TF*)c->flux[0], (const TF*)c->flux[1], (const TF*)c->flux[2], c->nf[0], c->nf[1], c->nf[2]};
     launch(c, MHD_K_UPDATE, blocks(c->ninterior), dim3(kBlock), k_update<NST, ST, TF>, c->d, c->p.gam - 1.,
-           (const double*)(c->scal + SC_TAU), s, so, pv, F, (const double*)c->dx, (const double*)c->dy,
-           (const double*)c->dz, c->ninterior);
+           (const double*)(c->scal + SC_TAU), s, so, pv, F, sdx(c), sdy(c), sdz(c), c->ninterior);
     CFaces b{c->bi, c->bj, c->bk};
     Faces bo{out[8], out[9], out[10]};
     CFaces bp{c->bip, c->bjp, c->bkp};
     const long nct = (long)(c->d.nx0 + 1) * c->d.ny0 * c->d.nz0 + (long)c->d.nx0 * (c->d.ny0 + 1) * c->d.nz0 +
                      (long)c->d.nx0 * c->d.ny0 * (c->d.nz0 + 1);
-    launch(c, MHD_K_CT, blocks(nct), dim3(kBlock), k_ct<NST, ST, TF>, c->d, (const double*)(c->scal + SC_TAU), b, bo,
-           bp, (const TF*)c->emf[0], (const TF*)c->emf[1], (const TF*)c->emf[2], (const double*)c->dx,
+    const CtFactor ctf{{c->met.ct[0], c->met.ct[1], c->met.ct[2]}};
+    launch(c, MHD_K_CT, blocks(nct), dim3(kBlock), k_ct<NST, ST, TF>, c->d, (const double*)(c->scal + SC_TAU), ctf, b,
+           bo, bp, (const TF*)c->emf[0], (const TF*)c->emf[1], (const TF*)c->emf[2], (const double*)c->dx,
            (const double*)c->dy, (const double*)c->dz);
 }
 
@@ -401,7 +510,8 @@ int mhd_eijk_lfm(mhd_ctx* c) {
         do_bxyz(c, false, true, false);
         launch_emf_fp32(c, sh_src(c));
     } else {
-        launch(c, MHD_K_BXYZ, blocks(c->ncell), dim3(kBlock), k_vf, cur_state(c), c->vf, c->ncell);
+        if (c->met.on) do_bxyz(c, true, false, false);   // vf and the face-normal velocities
+        else launch(c, MHD_K_BXYZ, blocks(c->ncell), dim3(kBlock), k_vf, cur_state(c), c->vf, c->ncell);
         launch_emf_fp64(c, cur_src(c));
     }
     return sync_status(c);
@@ -474,8 +584,12 @@ static void step_ab(mhd_ctx* c) {
         // extrapolated state straight into the fp32 stencil copies (sh order: .. bz vf bi bj bk)
         for (int a = 0; a < 8; ++a) s.xh[a] = c->sh[a];
         for (int a = 8; a < 11; ++a) s.xh[a] = c->sh[a + 1];
+        Proj<float> pj{};
+        pj.on = c->met.on ? 1 : 0;
+        for (int q = 0; q < 9; ++q) pj.n[q] = c->met.n[q / 3][q % 3];
+        for (int d = 0; d < 3; ++d) pj.w[d] = c->w32[d];
         launch(c, MHD_K_PREV, blocks(c->nmax), dim3(kBlock), k_predict_ab<float>, s,
-               (const double*)(c->scal + SC_TAU), c->sh[8], nc, nbi, nbj, nbk);
+               (const double*)(c->scal + SC_TAU), c->sh[8], pj, nc, nbi, nbj, nbk);
         launch(c, MHD_K_PREV, dim3(1), dim3(64), k_tau0_from_tau, c->scal + SC_TAU);
         do_ops32(c);
     } else {
@@ -485,8 +599,12 @@ static void step_ab(mhd_ctx* c) {
             s.xp[a] = c->xp[a];
             s.xh[a] = c->xh[a];
         }
+        Proj<double> pj{};
+        pj.on = c->met.on ? 1 : 0;
+        for (int q = 0; q < 9; ++q) pj.n[q] = c->met.n[q / 3][q % 3];
+        for (int d = 0; d < 3; ++d) pj.w[d] = c->wh64[d];
         launch(c, MHD_K_PREV, blocks(c->nmax), dim3(kBlock), k_predict_ab<double>, s,
-               (const double*)(c->scal + SC_TAU), c->vfh, nc, nbi, nbj, nbk);
+               (const double*)(c->scal + SC_TAU), c->vfh, pj, nc, nbi, nbj, nbk);
         launch(c, MHD_K_PREV, dim3(1), dim3(64), k_tau0_from_tau, c->scal + SC_TAU);
         do_ops64(c, h_src(c));
     }

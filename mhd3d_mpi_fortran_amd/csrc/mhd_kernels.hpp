@@ -108,8 +108,72 @@ __device__ __forceinline__ void face_flux(const Coef<R>& c, const R* l, const R*
     F[7 * nface + t] = TF(sz + tz);
 }
 
-template <typename R, int M, int D, typename TI, typename TF>
-__global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<TI> q, TF* __restrict__ F, long nface) {
+// ------------------------------------------------------------------------
+// affine metric (no counterpart in the reference, whose grid is a tensor
+// product, mhd3d3.f90:3483-3485): physical x = A * (computational x), the
+// computational grid being the tensor-product one.  Face family d has the
+// unit normal n_d = C e_d / |C e_d| (C = cofactor matrix of A); the flux
+// through it is the Gaussian-split flux in vector form along n_d.  With n_d
+// a coordinate axis this is the reference's get_flux_stress term for term.
+// ------------------------------------------------------------------------
+template <typename R>
+struct Normals {
+    R n[3][3];  // n[d] = unit normal of the faces normal to computational axis d
+};
+
+template <typename R, typename TF>
+__device__ __forceinline__ void face_flux_met(const Coef<R>& c, const R* l, const R* r, const R* n,
+                                              TF* __restrict__ F, long t, long nface) {
+    R v0, v1;
+    R fx[8];
+    // left state, positive half-moments
+    {
+        const R rl = l[0], ul = l[1], vl = l[2], wl = l[3], pl = l[4], bxl = l[5], byl = l[6], bzl = l[7];
+        const R un = n[0] * ul + n[1] * vl + n[2] * wl;
+        const R leng = R(.5) * rl * (sq(ul) + sq(vl) + sq(wl)) + pl / c.gm1;
+        R lam = rl / (R(2) * pl);
+        vface<R>(lam, un, +1, c.rspi, v0, v1);
+        fx[0] = rl * v1;
+        fx[1] = rl * ul * v1 + pl * n[0] * v0;
+        fx[2] = rl * vl * v1 + pl * n[1] * v0;
+        fx[3] = rl * wl * v1 + pl * n[2] * v0;
+        fx[4] = (leng + R(.5) * pl) * v1 + R(.5) * pl * un * v0;
+        const R bsq = sq(bxl) + sq(byl) + sq(bzl);
+        lam = rl / (R(2) * pl + bsq);
+        vface<R>(lam, un, +1, c.rspi, v0, v1);
+        const R bn = n[0] * bxl + n[1] * byl + n[2] * bzl;
+        fx[5] = (R(.5) * bsq * n[0] - bxl * bn) * v0;
+        fx[6] = (R(.5) * bsq * n[1] - byl * bn) * v0;
+        fx[7] = (R(.5) * bsq * n[2] - bzl * bn) * v0;
+    }
+    // right state, negative half-moments
+    {
+        const R rr = r[0], ur = r[1], vr = r[2], wr = r[3], pr = r[4], bxr = r[5], byr = r[6], bzr = r[7];
+        const R un = n[0] * ur + n[1] * vr + n[2] * wr;
+        const R leng = R(.5) * rr * (sq(ur) + sq(vr) + sq(wr)) + pr / c.gm1;
+        R lam = rr / (R(2) * pr);
+        vface<R>(lam, un, -1, c.rspi, v0, v1);
+        fx[0] += rr * v1;
+        fx[1] += rr * ur * v1 + pr * n[0] * v0;
+        fx[2] += rr * vr * v1 + pr * n[1] * v0;
+        fx[3] += rr * wr * v1 + pr * n[2] * v0;
+        fx[4] += (leng + R(.5) * pr) * v1 + R(.5) * pr * un * v0;
+        const R bsq = sq(bxr) + sq(byr) + sq(bzr);
+        lam = rr / (R(2) * pr + bsq);
+        vface<R>(lam, un, -1, c.rspi, v0, v1);
+        const R bn = n[0] * bxr + n[1] * byr + n[2] * bzr;
+        fx[5] += (R(.5) * bsq * n[0] - bxr * bn) * v0;
+        fx[6] += (R(.5) * bsq * n[1] - byr * bn) * v0;
+        fx[7] += (R(.5) * bsq * n[2] - bzr * bn) * v0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) F[q * nface + t] = TF(fx[q]);
+}
+
+// MET = false: the reference's axis-aligned flux; true: along the metric's normals
+template <typename R, int M, int D, typename TI, typename TF, bool MET>
+__global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<TI> q, Normals<R> nm, TF* __restrict__ F,
+                                                 long nface) {
     const long t = gtid();
     if (t >= nface) return;
     const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
@@ -126,7 +190,8 @@ __global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<TI> q, 
     recon_line<R, M>(q.bx, base, st, c, l[5], r[5]);
     recon_line<R, M>(q.by, base, st, c, l[6], r[6]);
     recon_line<R, M>(q.bz, base, st, c, l[7], r[7]);
-    face_flux<R, D, TF>(c, l, r, F, t, nface);
+    if (MET) face_flux_met<R, TF>(c, l, r, nm.n[D - 1], F, t, nface);
+    else face_flux<R, D, TF>(c, l, r, F, t, nface);
 }
 
 // ------------------------------------------------------------------------

@@ -9,10 +9,24 @@
 #include "mhd_kernels.hpp"
 
 namespace mhd {
-// stencil inputs of one state: cell primitives, cell B, vf, face B
+// stencil inputs of one state: cell primitives, cell B, vf, face B and,
+// with a metric, the face-normal velocities w_d = v . n_d
 template <typename T>
 struct Src {
     const T *rho, *vx, *vy, *vz, *p, *bx, *by, *bz, *vf, *bi, *bj, *bk;
+    const T* w[3];
+};
+
+// affine metric x_phys = A x_comp (mhd_set_metric); on = false is the
+// reference's tensor-product grid, for which every factor below is exactly 1
+struct Metric {
+    bool on = false;
+    double A[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};  // row-major
+    double n[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};  // unit face normals per computational axis
+    double g[3] = {1, 1, 1};    // |C e_d| / det A: flux divergence factor (update, CFL)
+    double ct[3] = {1, 1, 1};   // 1 / |C e_d|: constrained-transport factor of face family d
+    double ef[3] = {1, 1, 1};   // |C e_a||C e_b| / det A: EMF factor of edges along d
+    double M[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};  // A diag(g): face-average B -> cell B
 };
 }  // namespace mhd
 
@@ -66,6 +80,15 @@ struct mhd_ctx {
     std::vector<hipEvent_t> evpool;
     double prof_ms[MHD_K_NCLASS] = {};
     int prof_n[MHD_K_NCLASS] = {};
+    // metric (mhd_set_metric): spacing divided by g_d for the update and the
+    // CFL, face-normal velocities (binary64 current / extrapolated state, fp32)
+    mhd::Metric met;
+    std::vector<double> hdx, hdy, hdz;
+    double *dxm = nullptr, *dym = nullptr, *dzm = nullptr;
+    char* metbuf = nullptr;
+    double* w64[3] = {};
+    double* wh64[3] = {};
+    float* w32[3] = {};
     SurrogateState* sur = nullptr;
     float sur_theta = 0.25f;   // smoothness threshold of the learned reconstruction
     bool sur_stats = false;    // count faces / fallbacks
@@ -114,6 +137,14 @@ inline mhd::Coef<R> make_coef(const mhd_ctx* c) {
     k.alpha = R(c->p.alpha);
     k.beta = R(c->p.beta);
     k.rspi = R(std::sqrt(pi));
+    return k;
+}
+
+template <typename R>
+inline mhd::Normals<R> make_normals(const mhd_ctx* c) {
+    mhd::Normals<R> k;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) k.n[a][b] = R(c->met.n[a][b]);
     return k;
 }
 

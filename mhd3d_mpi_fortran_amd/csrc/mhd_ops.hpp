@@ -8,16 +8,23 @@
 
 namespace mhd {
 
-template <typename R, typename TI, typename TF, int M>
-void flux_launch_m(mhd_ctx* c, Src<TI> s) {
+template <typename R, typename TI, typename TF, int M, bool MET>
+void flux_launch_mm(mhd_ctx* c, Src<TI> s) {
     const Coef<R> co = make_coef<R>(c);
     const Prim<TI> q{s.rho, s.vx, s.vy, s.vz, s.p, s.bx, s.by, s.bz};
-    launch(c, MHD_K_FLUX, blocks(c->nf[0]), dim3(kBlock), k_flux<R, M, 1, TI, TF>, c->d, co, q, (TF*)c->flux[0],
-           c->nf[0]);
-    launch(c, MHD_K_FLUX, blocks(c->nf[1]), dim3(kBlock), k_flux<R, M, 2, TI, TF>, c->d, co, q, (TF*)c->flux[1],
-           c->nf[1]);
-    launch(c, MHD_K_FLUX, blocks(c->nf[2]), dim3(kBlock), k_flux<R, M, 3, TI, TF>, c->d, co, q, (TF*)c->flux[2],
-           c->nf[2]);
+    const Normals<R> nm = make_normals<R>(c);
+    launch(c, MHD_K_FLUX, blocks(c->nf[0]), dim3(kBlock), k_flux<R, M, 1, TI, TF, MET>, c->d, co, q, nm,
+           (TF*)c->flux[0], c->nf[0]);
+    launch(c, MHD_K_FLUX, blocks(c->nf[1]), dim3(kBlock), k_flux<R, M, 2, TI, TF, MET>, c->d, co, q, nm,
+           (TF*)c->flux[1], c->nf[1]);
+    launch(c, MHD_K_FLUX, blocks(c->nf[2]), dim3(kBlock), k_flux<R, M, 3, TI, TF, MET>, c->d, co, q, nm,
+           (TF*)c->flux[2], c->nf[2]);
+}
+
+template <typename R, typename TI, typename TF, int M>
+void flux_launch_m(mhd_ctx* c, Src<TI> s) {
+    if (c->met.on) flux_launch_mm<R, TI, TF, M, true>(c, s);
+    else flux_launch_mm<R, TI, TF, M, false>(c, s);
 }
 
 template <typename R, typename TI, typename TF>
@@ -45,15 +52,22 @@ void emf_launch_k(mhd_ctx* c, EmfIn<TI> in) {
     const long nm = K == 3 ? d.ny0 + 1 : d.nz0 + 1;
     const long n = ii * no * ((nm + T - 1) / T);
     launch(c, MHD_K_EMF, blocks(n), dim3(kBlock), k_emf_march<R, M, K, T, TI, TF>, d, make_coef<R>(c), in,
-           (TF*)c->emf[K - 1], n);
+           R(c->met.ef[K - 1]), (TF*)c->emf[K - 1], n);
 }
 
-// edge k: (v1, v2, b1, b2) = (vy, vz, bj, bk), (vz, vx, bk, bi), (vx, vy, bi, bj)
+// edge k: (v1, v2, b1, b2) = (vy, vz, bj, bk), (vz, vx, bk, bi), (vx, vy, bi, bj);
+// with a metric the velocities are the face-normal ones w_d = v . n_d
 template <typename R, typename TI, typename TF, int M>
 void emf_launch_m(mhd_ctx* c, Src<TI> s) {
-    emf_launch_k<R, M, 1, TI, TF>(c, EmfIn<TI>{s.vy, s.vz, s.bj, s.bk, s.vf});
-    emf_launch_k<R, M, 2, TI, TF>(c, EmfIn<TI>{s.vz, s.vx, s.bk, s.bi, s.vf});
-    emf_launch_k<R, M, 3, TI, TF>(c, EmfIn<TI>{s.vx, s.vy, s.bi, s.bj, s.vf});
+    const TI* u[3] = {s.vx, s.vy, s.vz};
+    if (c->met.on) {
+        u[0] = s.w[0];
+        u[1] = s.w[1];
+        u[2] = s.w[2];
+    }
+    emf_launch_k<R, M, 1, TI, TF>(c, EmfIn<TI>{u[1], u[2], s.bj, s.bk, s.vf});
+    emf_launch_k<R, M, 2, TI, TF>(c, EmfIn<TI>{u[2], u[0], s.bk, s.bi, s.vf});
+    emf_launch_k<R, M, 3, TI, TF>(c, EmfIn<TI>{u[0], u[1], s.bi, s.bj, s.vf});
 }
 
 template <typename R, typename TI, typename TF>

@@ -116,11 +116,18 @@ struct CFaces {
     const double *bi, *bj, *bk;
 };
 
+// ctf[d] = 1/|C e_d| of a metric per face family, exactly 1 on the
+// reference's tensor-product grid (then tau * ctf == tau bit for bit)
+struct CtFactor {
+    double f[3];
+};
+
 template <int NST, int ST, typename TF>
-__global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict__ tau_p, CFaces b, Faces bo, CFaces bp,
-                                              const TF* __restrict__ ei, const TF* __restrict__ ej,
-                                              const TF* __restrict__ ek, const double* __restrict__ dx,
-                                              const double* __restrict__ dy, const double* __restrict__ dz) {
+__global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict__ tau_p, CtFactor ctf, CFaces b,
+                                              Faces bo, CFaces bp, const TF* __restrict__ ei,
+                                              const TF* __restrict__ ej, const TF* __restrict__ ek,
+                                              const double* __restrict__ dx, const double* __restrict__ dy,
+                                              const double* __restrict__ dz) {
     const long t = gtid();
     const int nx0 = d.nx0, ny0 = d.ny0, nz0 = d.nz0, ng = d.ng;
     const long nbi = (long)(nx0 + 1) * ny0 * nz0, nbj = (long)nx0 * (ny0 + 1) * nz0,
@@ -140,7 +147,7 @@ __global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict_
         const double dej = double(ej[e + (ptrdiff_t)(nx0 + 1) * ny0]) - double(ej[e]);
         q = ix(d.nx + 1, d.ny, ng + i, ng + j, ng + k);
         old = b.bi[q];
-        val = old - tau * (dek / dy[ng + j - 1] - dej / dz[ng + k - 1]);
+        val = old - tau * ctf.f[0] * (dek / dy[ng + j - 1] - dej / dz[ng + k - 1]);
         out = bo.bi;
         prev = bp.bi;
     } else if (t < nbi + nbj) {
@@ -151,7 +158,7 @@ __global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict_
         const double dek = double(ek[e + 1]) - double(ek[e]);
         q = ix(d.nx, d.ny + 1, ng + i, ng + j, ng + k);
         old = b.bj[q];
-        val = old - tau * (dei / dz[ng + k - 1] - dek / dx[ng + i - 1]);
+        val = old - tau * ctf.f[1] * (dei / dz[ng + k - 1] - dek / dx[ng + i - 1]);
         out = bo.bj;
         prev = bp.bj;
     } else {
@@ -162,7 +169,7 @@ __global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict_
         const double dei = double(ei[e + nx0]) - double(ei[e]);
         q = ix(d.nx, d.ny, ng + i, ng + j, ng + k);
         old = b.bk[q];
-        val = old - tau * (dej / dx[ng + i - 1] - dei / dy[ng + j - 1]);
+        val = old - tau * ctf.f[2] * (dej / dx[ng + i - 1] - dei / dy[ng + j - 1]);
         out = bo.bk;
         prev = bp.bk;
     }
@@ -277,6 +284,16 @@ struct Shadow {  // fp32 stencil inputs of one state
     float *rho, *vx, *vy, *vz, *p, *bx, *by, *bz, *vf, *bi, *bj, *bk;
 };
 
+// metric part of get_bxyz: cell B = M (face averages), M = A diag(g), and
+// the face-normal velocities w_d = v . n_d (binary64 and/or fp32 copies)
+struct BMet {
+    int on;
+    double M[9];
+    double n[9];
+    double* w[3];
+    float* ws[3];
+};
+
 __device__ __forceinline__ double cfl_dt(double gam, double cfl, double u, double v, double w, double x, double y,
                                          double z, double r, double pp, double dxi, double dyj, double dzk) {
     const double vv = sqrt(u * u + v * v + w * w);
@@ -289,7 +306,7 @@ __device__ __forceinline__ double cfl_dt(double gam, double cfl, double u, doubl
 
 __global__ void __launch_bounds__(kBlock) k_bxyz(Dims d, double gam, double cfl, CFaces b, CCells s,
                                                 double* __restrict__ bx, double* __restrict__ by,
-                                                double* __restrict__ bz, double* __restrict__ vf, Shadow sh,
+                                                double* __restrict__ bz, double* __restrict__ vf, Shadow sh, BMet met,
                                                 const double* __restrict__ dx, const double* __restrict__ dy,
                                                 const double* __restrict__ dz, double* __restrict__ part, long ncell,
                                                 long nbi, long nbj, long nbk) {
@@ -298,13 +315,25 @@ __global__ void __launch_bounds__(kBlock) k_bxyz(Dims d, double gam, double cfl,
     if (t < ncell) {
         int i, j, k;
         unflat(t, d.nx, d.ny, i, j, k);
-        const double x = .5 * (b.bi[ix(d.nx + 1, d.ny, i + 1, j, k)] + b.bi[ix(d.nx + 1, d.ny, i, j, k)]);
-        const double y = .5 * (b.bj[ix(d.nx, d.ny + 1, i, j + 1, k)] + b.bj[ix(d.nx, d.ny + 1, i, j, k)]);
-        const double z = .5 * (b.bk[t + (ptrdiff_t)d.nx * d.ny] + b.bk[t]);
+        double x = .5 * (b.bi[ix(d.nx + 1, d.ny, i + 1, j, k)] + b.bi[ix(d.nx + 1, d.ny, i, j, k)]);
+        double y = .5 * (b.bj[ix(d.nx, d.ny + 1, i, j + 1, k)] + b.bj[ix(d.nx, d.ny + 1, i, j, k)]);
+        double z = .5 * (b.bk[t + (ptrdiff_t)d.nx * d.ny] + b.bk[t]);
+        const double u = s.vx[t], v = s.vy[t], w = s.vz[t], r = s.rho[t], pp = s.p[t];
+        if (met.on) {
+            const double fx = x, fy = y, fz = z;
+            x = met.M[0] * fx + met.M[1] * fy + met.M[2] * fz;
+            y = met.M[3] * fx + met.M[4] * fy + met.M[5] * fz;
+            z = met.M[6] * fx + met.M[7] * fy + met.M[8] * fz;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const double wn = met.n[3 * a] * u + met.n[3 * a + 1] * v + met.n[3 * a + 2] * w;
+                if (met.w[a]) met.w[a][t] = wn;
+                if (met.ws[a]) met.ws[a][t] = float(wn);
+            }
+        }
         bx[t] = x;
         by[t] = y;
         bz[t] = z;
-        const double u = s.vx[t], v = s.vy[t], w = s.vz[t], r = s.rho[t], pp = s.p[t];
         if (vf || sh.rho) {
             const double f = sqrt(u * u + v * v + w * w) + sqrt((x * x + y * y + z * z) / r);
             if (vf) vf[t] = f;
@@ -430,10 +459,18 @@ struct AB11 {
     TH* xh[11];
 };
 
+// face-normal velocities of the extrapolated state (metric only)
+template <typename TH>
+struct Proj {
+    int on;
+    double n[9];
+    TH* w[3];
+};
+
 template <typename TH>
 __global__ void __launch_bounds__(kBlock) k_predict_ab(AB11<TH> s, const double* __restrict__ tau_p,
-                                                      TH* __restrict__ vfh, long ncell, long nbi, long nbj,
-                                                      long nbk) {
+                                                      TH* __restrict__ vfh, Proj<TH> pj, long ncell, long nbi,
+                                                      long nbj, long nbk) {
     const long t = gtid();
     const double c = tau_p[0] / tau_p[1] / 2.;
     if (t < ncell) {
@@ -445,6 +482,10 @@ __global__ void __launch_bounds__(kBlock) k_predict_ab(AB11<TH> s, const double*
             s.xh[a][t] = TH(h[a]);
         }
         vfh[t] = TH(sqrt(h[1] * h[1] + h[2] * h[2] + h[3] * h[3]) + sqrt((h[5] * h[5] + h[6] * h[6] + h[7] * h[7]) / h[0]));
+        if (pj.on) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) pj.w[a][t] = TH(pj.n[3 * a] * h[1] + pj.n[3 * a + 1] * h[2] + pj.n[3 * a + 2] * h[3]);
+        }
     }
     const long nf[3] = {nbi, nbj, nbk};
 #pragma unroll

@@ -470,8 +470,8 @@ __global__ void __launch_bounds__(kBlock) k_infer(const float* __restrict__ x, l
 // ---------------------------------------------------------------------------
 constexpr int SUR_GROUPS = 4;
 
-template <int D>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_flux_sur(Dims d, Coef<float> c, Prim<float> q, float theta,
+template <int D, bool MET>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_flux_sur(Dims d, Coef<float> c, Prim<float> q, Normals<float> nm, float theta,
                                                     const unsigned short* __restrict__ wbf,
                                                     const float* __restrict__ bias, float* __restrict__ F, long nface,
                                                     int* __restrict__ fix, unsigned long long* __restrict__ nfix,
@@ -543,13 +543,16 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
                 atomicAdd(&stats[1], (unsigned long long)__popcll(fm));
             }
         }
-        if (live && !fb) face_flux<float, D>(c, lv, rv, F, t, nface);
+        if (live && !fb) {
+            if (MET) face_flux_met<float, float>(c, lv, rv, nm.n[D - 1], F, t, nface);
+            else face_flux<float, D>(c, lv, rv, F, t, nface);
+        }
     }
 }
 
 // PDM flux of the listed fallback faces (fp32, as the plain mixed path)
-template <int D>
-__global__ void __launch_bounds__(kBlock) k_flux_fix(Dims d, Coef<float> c, Prim<float> q,
+template <int D, bool MET>
+__global__ void __launch_bounds__(kBlock) k_flux_fix(Dims d, Coef<float> c, Prim<float> q, Normals<float> nm,
                                                     const int* __restrict__ fix,
                                                     const unsigned long long* __restrict__ nfix,
                                                     float* __restrict__ F, long nface) {
@@ -570,7 +573,8 @@ __global__ void __launch_bounds__(kBlock) k_flux_fix(Dims d, Coef<float> c, Prim
         recon_line<float, PDM>(q.bx, base, st, c, l[5], r[5]);
         recon_line<float, PDM>(q.by, base, st, c, l[6], r[6]);
         recon_line<float, PDM>(q.bz, base, st, c, l[7], r[7]);
-        face_flux<float, D>(c, l, r, F, t, nface);
+        if (MET) face_flux_met<float, float>(c, l, r, nm.n[D - 1], F, t, nface);
+        else face_flux<float, D>(c, l, r, F, t, nface);
     }
 }
 
@@ -622,6 +626,36 @@ void surrogate_free(mhd_ctx* c) {
     c->sur = nullptr;
 }
 
+template <bool MET>
+static void flux_sur_launch(mhd_ctx* c, SurrogateState* S, Src<float> s) {
+    const Coef<float> co = make_coef<float>(c);
+    const Normals<float> nm = make_normals<float>(c);
+    const Prim<float> q{s.rho, s.vx, s.vy, s.vz, s.p, s.bx, s.by, s.bz};
+    const float theta = c->sur_theta;
+    unsigned long long* st = c->sur_stats ? S->stats : nullptr;
+    launch(c, MHD_K_FLUX, dim3(1), dim3(64), sur::k_zero_u64, S->nfix, 3);
+    int* fx[3] = {S->fix, S->fix + c->nf[0], S->fix + c->nf[0] + c->nf[1]};
+    const unsigned gsur[3] = {(unsigned)((c->nf[0] + sur::SUR_GROUPS * kBlock - 1) / (sur::SUR_GROUPS * kBlock)),
+                              (unsigned)((c->nf[1] + sur::SUR_GROUPS * kBlock - 1) / (sur::SUR_GROUPS * kBlock)),
+                              (unsigned)((c->nf[2] + sur::SUR_GROUPS * kBlock - 1) / (sur::SUR_GROUPS * kBlock))};
+    const unsigned short* wbf = S->wbf;
+    const float* bias = S->bias;
+    launch(c, MHD_K_FLUX, dim3(gsur[0]), dim3(kBlock), sur::k_flux_sur<1, MET>, c->d, co, q, nm, theta, wbf, bias,
+           (float*)c->flux[0], c->nf[0], fx[0], S->nfix + 0, st);
+    launch(c, MHD_K_FLUX, dim3(gsur[1]), dim3(kBlock), sur::k_flux_sur<2, MET>, c->d, co, q, nm, theta, wbf, bias,
+           (float*)c->flux[1], c->nf[1], fx[1], S->nfix + 1, st);
+    launch(c, MHD_K_FLUX, dim3(gsur[2]), dim3(kBlock), sur::k_flux_sur<3, MET>, c->d, co, q, nm, theta, wbf, bias,
+           (float*)c->flux[2], c->nf[2], fx[2], S->nfix + 2, st);
+    // the fallback lists: a fixed grid walks whatever length each list has
+    const dim3 gfix(1024);
+    launch(c, MHD_K_FLUX, gfix, dim3(kBlock), sur::k_flux_fix<1, MET>, c->d, co, q, nm, (const int*)fx[0],
+           (const unsigned long long*)(S->nfix + 0), (float*)c->flux[0], c->nf[0]);
+    launch(c, MHD_K_FLUX, gfix, dim3(kBlock), sur::k_flux_fix<2, MET>, c->d, co, q, nm, (const int*)fx[1],
+           (const unsigned long long*)(S->nfix + 1), (float*)c->flux[1], c->nf[1]);
+    launch(c, MHD_K_FLUX, gfix, dim3(kBlock), sur::k_flux_fix<3, MET>, c->d, co, q, nm, (const int*)fx[2],
+           (const unsigned long long*)(S->nfix + 2), (float*)c->flux[2], c->nf[2]);
+}
+
 void launch_flux_surrogate(mhd_ctx* c, Src<float> s) {
     SurrogateState* S = c->sur;
     if (!S || !S->have_weights) {  // no trained net: plain fp32 PDM
@@ -639,31 +673,8 @@ void launch_flux_surrogate(mhd_ctx* c, Src<float> s) {
             return;
         }
     }
-    const Coef<float> co = make_coef<float>(c);
-    const Prim<float> q{s.rho, s.vx, s.vy, s.vz, s.p, s.bx, s.by, s.bz};
-    const float theta = c->sur_theta;
-    unsigned long long* st = c->sur_stats ? S->stats : nullptr;
-    launch(c, MHD_K_FLUX, dim3(1), dim3(64), sur::k_zero_u64, S->nfix, 3);
-    int* fx[3] = {S->fix, S->fix + c->nf[0], S->fix + c->nf[0] + c->nf[1]};
-    const unsigned gsur[3] = {(unsigned)((c->nf[0] + sur::SUR_GROUPS * kBlock - 1) / (sur::SUR_GROUPS * kBlock)),
-                              (unsigned)((c->nf[1] + sur::SUR_GROUPS * kBlock - 1) / (sur::SUR_GROUPS * kBlock)),
-                              (unsigned)((c->nf[2] + sur::SUR_GROUPS * kBlock - 1) / (sur::SUR_GROUPS * kBlock))};
-    const unsigned short* wbf = S->wbf;
-    const float* bias = S->bias;
-    launch(c, MHD_K_FLUX, dim3(gsur[0]), dim3(kBlock), sur::k_flux_sur<1>, c->d, co, q, theta, wbf, bias,
-           (float*)c->flux[0], c->nf[0], fx[0], S->nfix + 0, st);
-    launch(c, MHD_K_FLUX, dim3(gsur[1]), dim3(kBlock), sur::k_flux_sur<2>, c->d, co, q, theta, wbf, bias,
-           (float*)c->flux[1], c->nf[1], fx[1], S->nfix + 1, st);
-    launch(c, MHD_K_FLUX, dim3(gsur[2]), dim3(kBlock), sur::k_flux_sur<3>, c->d, co, q, theta, wbf, bias,
-           (float*)c->flux[2], c->nf[2], fx[2], S->nfix + 2, st);
-    // the fallback lists: a fixed grid walks whatever length each list has
-    const dim3 gfix(1024);
-    launch(c, MHD_K_FLUX, gfix, dim3(kBlock), sur::k_flux_fix<1>, c->d, co, q, (const int*)fx[0],
-           (const unsigned long long*)(S->nfix + 0), (float*)c->flux[0], c->nf[0]);
-    launch(c, MHD_K_FLUX, gfix, dim3(kBlock), sur::k_flux_fix<2>, c->d, co, q, (const int*)fx[1],
-           (const unsigned long long*)(S->nfix + 1), (float*)c->flux[1], c->nf[1]);
-    launch(c, MHD_K_FLUX, gfix, dim3(kBlock), sur::k_flux_fix<3>, c->d, co, q, (const int*)fx[2],
-           (const unsigned long long*)(S->nfix + 2), (float*)c->flux[2], c->nf[2]);
+    if (c->met.on) flux_sur_launch<true>(c, S, s);
+    else flux_sur_launch<false>(c, S, s);
 }
 
 static int sync_or(mhd_ctx* c) {

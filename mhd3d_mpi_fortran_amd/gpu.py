@@ -47,6 +47,7 @@ SIGNATURES = {
     "mhd_destroy": (None, [C.c_void_p]),
     "mhd_error_string": (C.c_char_p, [C.c_int]),
     "mhd_set_geometry": (C.c_int, [C.c_void_p, _D, _D, _D]),
+    "mhd_set_metric": (C.c_int, [C.c_void_p, _D]),
     "mhd_upload_state": (C.c_int, [C.c_void_p] + [_D] * 8),
     "mhd_download_state": (C.c_int, [C.c_void_p] + [_D] * 11),
     "mhd_initialize": (C.c_int, [C.c_void_p, _D]),
@@ -141,6 +142,13 @@ class Solver:
         dy = np.ascontiguousarray(grid.dy)
         dz = np.ascontiguousarray(grid.dz)
         _ck(lib().mhd_set_geometry(self.h, _p(dx), _p(dy), _p(dz)), "mhd_set_geometry")
+        if getattr(grid, "A", None) is not None:
+            self.set_metric(grid.A)
+
+    def set_metric(self, A):
+        """affine metric x_phys = A x_comp (row-major 3x3)"""
+        a = np.ascontiguousarray(np.asarray(A, dtype=np.float64).reshape(3, 3))
+        _ck(lib().mhd_set_metric(self.h, _p(a)), "mhd_set_metric")
 
     def close(self):
         if getattr(self, "h", None):

@@ -15,6 +15,10 @@ bench, the Fortran driver's IC file -- builds the same bits):
   * ``blast``      seeded blast wave, 4 centres, p=100 inside r<radius;
   * ``blobs``      rho = 1 + three seeded Gaussian blobs, p=1, uniform |B|=0.5;
   * ``rough``      uniform random fields (exercises every limiter branch).
+
+A grid may carry an affine metric ``A`` (physical x = A x_comp, e.g. a
+sheared box): positions are then physical, vectors are Cartesian, and the
+face arrays hold the normal components B.n of each face family.
 """
 import math
 
@@ -78,7 +82,8 @@ def _c4(a, b, pattern):
 class Grid:
     """Block geometry of one domain (whole domain on one device)."""
 
-    def __init__(self, nx0, ny0, nz0, ng=4, lo=(0., 0., 0.), hi=(1., 1., 1.)):
+    def __init__(self, nx0, ny0, nz0, ng=4, lo=(0., 0., 0.), hi=(1., 1., 1.), A=None):
+        self.A = None if A is None else np.asarray(A, dtype=np.float64).reshape(3, 3)
         self.nx0, self.ny0, self.nz0, self.ng = nx0, ny0, nz0, ng
         self.nx, self.ny, self.nz = nx0 + 2 * ng, ny0 + 2 * ng, nz0 + 2 * ng
         self.lo, self.hi = tuple(lo), tuple(hi)
@@ -86,6 +91,21 @@ class Grid:
         self.yf = axis_faces(lo[1], hi[1], ny0, ng)
         self.zf = axis_faces(lo[2], hi[2], nz0, ng)
         self.dx, self.dy, self.dz = metric(self.xf), metric(self.yf), metric(self.zf)
+
+    def face_normals(self):
+        """unit normals n[d] of the faces normal to computational axis d"""
+        A = np.eye(3) if self.A is None else self.A
+        a = [A[:, d] for d in range(3)]
+        n = np.array([np.cross(a[(d + 1) % 3], a[(d + 2) % 3]) for d in range(3)])
+        return n / np.linalg.norm(n, axis=1)[:, None]
+
+    def physical(self, x, y, z):
+        """map computational coordinates (broadcastable arrays) to physical"""
+        if self.A is None:
+            return x, y, z
+        A = self.A
+        return (A[0, 0] * x + A[0, 1] * y + A[0, 2] * z, A[1, 0] * x + A[1, 1] * y + A[1, 2] * z,
+                A[2, 0] * x + A[2, 1] * y + A[2, 2] * z)
 
     # broadcastable 3-D coordinate arrays (Fortran order is applied at the end)
     def cell_centres(self):
@@ -147,6 +167,8 @@ def initial_state(g, kind, seed=0, **kw):
     pi = 4. * math.atan(1.)
     xc, yc, zc = g.cell_centres()
     st = {}
+    if g.A is not None and kind != "blast":
+        raise ValueError("initial state %r is defined on the tensor-product grid only" % kind)
     if kind == "shocktube":
         rho = _full(cs, 1.)
         p = _full(cs, 1.)
@@ -183,14 +205,21 @@ def initial_state(g, kind, seed=0, **kw):
         xb, yb, zb = (_bcast(c, cs) for c in (xc, yc, zc))
         inside = np.zeros(cs, dtype=bool)
         for c in cent:
-            d = [np.abs(xb - c[0]), np.abs(yb - c[1]), np.abs(zb - c[2])]
-            d = [np.minimum(dd, Lq - dd) for dd, Lq in zip(d, L)]   # periodic images
+            # nearest periodic image in computational coordinates, distance in
+            # physical ones (the same thing without a metric)
+            if g.A is None:
+                d = [np.abs(xb - c[0]), np.abs(yb - c[1]), np.abs(zb - c[2])]
+                d = [np.minimum(dd, Lq - dd) for dd, Lq in zip(d, L)]
+            else:
+                d = [xb - c[0], yb - c[1], zb - c[2]]
+                d = g.physical(*[dd - Lq * np.round(dd / Lq) for dd, Lq in zip(d, L)])
             inside |= (d[0] ** 2 + d[1] ** 2 + d[2] ** 2) < r0 * r0
         p = _full(cs, pout)
         p[inside] = pin
         b = 1. / math.sqrt(2.)
+        bn = g.face_normals() @ np.array([b, b, 0.])   # normal components of the uniform B
         st = dict(rho=_full(cs, 1.), vx=_full(cs), vy=_full(cs), vz=_full(cs), p=p,
-                  bi=_full(fi, b), bj=_full(fj, b), bk=_full(fk, 0.))
+                  bi=_full(fi, bn[0]), bj=_full(fj, bn[1]), bk=_full(fk, bn[2]))
     elif kind == "blobs":
         u = uniform01(seed, 3 * 5 + 2)
         lo, hi = np.array(g.lo), np.array(g.hi)
