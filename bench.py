@@ -2,10 +2,11 @@
 """Benchmark: zone-updates/s of the ideal-MHD step on one MI355X.
 
 Workload (largest single-GPU configuration of the benchmark set): a 256^3
-periodic box, gamma=5/3, Adams-Bashforth + constrained transport, PDM
-reconstruction, seeded blast wave (4 centres, p=100 inside r<0.1, p=0.1
-outside, rho=1, B=(1,1,0)/sqrt(2)).  Grid: uniform Cartesian (the
-reference's tensor-product geometry).
+periodic sheared box -- physical x = A x_comp with A = [[1, .5, 0], [0, 1, 0],
+[0, 0, 1]] over the unit cube, so the i-faces lean at atan(.5) -- gamma=5/3,
+Adams-Bashforth + constrained transport, PDM reconstruction, seeded blast
+wave (4 centres, p=100 inside r<0.1, p=0.1 outside, rho=1,
+B=(1,1,0)/sqrt(2)).
 
 Two runs from the same initial state:
   * plain: binary64 everywhere (parity-checked against the CPU oracle);
@@ -23,8 +24,9 @@ one full step.
 
 cpu_baseline: the reference solver (oracle/_ref, built from mhd3d3.f90) on
 the host cores with MPI, on a bounded sample of the same workload, timed by
-its own step-loop timer; falls back to the C oracle when the reference
-build or MPI is unavailable.
+its own step-loop timer; the reference has no sheared metric, so its sample
+is the same 256^3 blast wave on its Cartesian grid.  Falls back to the C
+oracle when the reference build or MPI is unavailable.
 """
 import argparse
 import json
@@ -62,8 +64,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def workload(n):
-    g = ics.Grid(n, n, n, ng=4)
+SHEAR = [[1., .5, 0.], [0., 1., 0.], [0., 0., 1.]]
+
+
+def workload(n, sheared=True):
+    g = ics.Grid(n, n, n, ng=4, A=SHEAR if sheared else None)
     st = ics.initial_state(g, "blast", seed=4, radius=.1, p_in=100., p_out=.1)
     return g, st
 
@@ -105,12 +110,19 @@ def l1_rel(a, b, g):
 
 
 def max_divb(st, g):
+    """max |div B| = |sum over faces of (B.n) area| / volume"""
     bi, bj, bk = g.interior(st["bi"], 1), g.interior(st["bj"], 2), g.interior(st["bk"], 3)
     ng = g.ng
     dx = g.dx[ng:ng + g.nx0][:, None, None]
     dy = g.dy[ng:ng + g.ny0][None, :, None]
     dz = g.dz[ng:ng + g.nz0][None, None, :]
-    div = (bi[1:] - bi[:-1]) / dx + (bj[:, 1:] - bj[:, :-1]) / dy + (bk[:, :, 1:] - bk[:, :, :-1]) / dz
+    fac = [1., 1., 1.]
+    if g.A is not None:
+        A = g.A
+        C = np.array([np.cross(A[:, (d + 1) % 3], A[:, (d + 2) % 3]) for d in range(3)])
+        fac = list(np.linalg.norm(C, axis=1) / np.linalg.det(A))
+    div = (fac[0] * (bi[1:] - bi[:-1]) / dx + fac[1] * (bj[:, 1:] - bj[:, :-1]) / dy +
+           fac[2] * (bk[:, :, 1:] - bk[:, :, :-1]) / dz)
     return float(np.abs(div).max())
 
 
@@ -153,7 +165,7 @@ def cpu_reference(g, st, n, nsteps, ranks):
             if "STEPLOOP_SECONDS" in line:
                 t = float(line.split()[-1])
                 return dict(value=n ** 3 * nsteps / t, unit="zone-updates/s", cores=ranks, kind="reference",
-                            sample="%d^3 blast wave, PDM AB2, %d steps, mpiexec -n %d (%dx%dx%d), step loop "
+                            sample="%d^3 blast wave on the reference's Cartesian grid, PDM AB2, %d steps, mpiexec -n %d (%dx%dx%d), step loop "
                                    "timed by MPI_Wtime" % (n, nsteps, ranks, px[0], px[1], px[2]))
         return None
     except (subprocess.TimeoutExpired, OSError) as e:
@@ -174,7 +186,7 @@ def cpu_port(g, st, n, nsteps, threads):
     o.step(nsteps, tau)
     t = time.perf_counter() - t0
     return dict(value=n ** 3 * nsteps / t, unit="zone-updates/s", cores=threads, kind="port",
-                sample="%d^3 blast wave, PDM AB2, %d steps, C oracle with %d OpenMP threads" % (n, nsteps, threads))
+                sample="%d^3 blast wave on the Cartesian grid, PDM AB2, %d steps, C oracle with %d OpenMP threads" % (n, nsteps, threads))
 
 
 def main():
@@ -250,9 +262,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 stencils + f64 accumulate" if prec == "mixed" else "f64",
         "data": "synthetic (seeded blast wave)",
-        "config": {"workload": "%d^3 periodic blast wave, AB2 + CT, PDM, gamma=5/3" % n,
+        "config": {"workload": "%d^3 periodic sheared-box blast wave, AB2 + CT, PDM, gamma=5/3" % n,
                    "mode": "mixed precision" + (" + learned reconstruction" if (sur and prec == "mixed") else ""),
-                   "grid": "uniform Cartesian", "parallelism": "single GPU, whole-domain block"},
+                   "grid": "sheared Cartesian, x = A x_comp, A = [[1,.5,0],[0,1,0],[0,0,1]]",
+                   "parallelism": "single GPU, whole-domain block"},
         "roofline": {"bound": "hbm", "kernel": kcls, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "avg_launch_ms": avg_s * 1e3, "alg_bytes_per_launch": alg},
@@ -281,10 +294,11 @@ def main():
         cores = host_cores()
         ranks = 16 if cores >= 16 else 8
         log("cpu baseline: reference with %d MPI ranks ..." % ranks)
-        cb = cpu_reference(g, st, n, args.cpu_steps, ranks)
+        gc, stc = workload(n, sheared=False)   # the reference's own (Cartesian) geometry
+        cb = cpu_reference(gc, stc, n, args.cpu_steps, ranks)
         if cb is None:
             m = 64
-            g2, st2 = workload(m)
+            g2, st2 = workload(m, sheared=False)
             cb = cpu_port(g2, st2, m, 2, cores)
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)

@@ -633,7 +633,7 @@ static void flux_sur_launch(mhd_ctx* c, SurrogateState* S, Src<float> s) {
     const Prim<float> q{s.rho, s.vx, s.vy, s.vz, s.p, s.bx, s.by, s.bz};
     const float theta = c->sur_theta;
     unsigned long long* st = c->sur_stats ? S->stats : nullptr;
-    launch(c, MHD_K_FLUX, dim3(1), dim3(64), sur::k_zero_u64, S->nfix, 3);
+    launch(c, MHD_K_SURR, dim3(1), dim3(64), sur::k_zero_u64, S->nfix, 3);
     int* fx[3] = {S->fix, S->fix + c->nf[0], S->fix + c->nf[0] + c->nf[1]};
     const unsigned gsur[3] = {(unsigned)((c->nf[0] + sur::SUR_GROUPS * kBlock - 1) / (sur::SUR_GROUPS * kBlock)),
                               (unsigned)((c->nf[1] + sur::SUR_GROUPS * kBlock - 1) / (sur::SUR_GROUPS * kBlock)),
@@ -648,11 +648,11 @@ static void flux_sur_launch(mhd_ctx* c, SurrogateState* S, Src<float> s) {
            (float*)c->flux[2], c->nf[2], fx[2], S->nfix + 2, st);
     // the fallback lists: a fixed grid walks whatever length each list has
     const dim3 gfix(1024);
-    launch(c, MHD_K_FLUX, gfix, dim3(kBlock), sur::k_flux_fix<1, MET>, c->d, co, q, nm, (const int*)fx[0],
+    launch(c, MHD_K_SURR, gfix, dim3(kBlock), sur::k_flux_fix<1, MET>, c->d, co, q, nm, (const int*)fx[0],
            (const unsigned long long*)(S->nfix + 0), (float*)c->flux[0], c->nf[0]);
-    launch(c, MHD_K_FLUX, gfix, dim3(kBlock), sur::k_flux_fix<2, MET>, c->d, co, q, nm, (const int*)fx[1],
+    launch(c, MHD_K_SURR, gfix, dim3(kBlock), sur::k_flux_fix<2, MET>, c->d, co, q, nm, (const int*)fx[1],
            (const unsigned long long*)(S->nfix + 1), (float*)c->flux[1], c->nf[1]);
-    launch(c, MHD_K_FLUX, gfix, dim3(kBlock), sur::k_flux_fix<3, MET>, c->d, co, q, nm, (const int*)fx[2],
+    launch(c, MHD_K_SURR, gfix, dim3(kBlock), sur::k_flux_fix<3, MET>, c->d, co, q, nm, (const int*)fx[2],
            (const unsigned long long*)(S->nfix + 2), (float*)c->flux[2], c->nf[2]);
 }
 
