@@ -60,6 +60,30 @@ BF16_PEAK_TFLOPS = 2500.0
 MLP_FLOPS = 2 * (40 * 32 + 32 * 32 + 32 * 16)
 
 
+# HBM bytes per launch of the stencil kernels, from the rocprofv3 FETCH_SIZE /
+# WRITE_SIZE passes committed under profiles/ (same workload)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "blast256_sheared_traffic.json")
+TRAFFIC_KERNELS = {("flux", "mixed", True): ["sur::k_flux_sur<%d, true>" % d for d in (1, 2, 3)],
+                   ("flux", "mixed", False): ["mhd::k_flux<float, 1, %d, float, float, true>" % d for d in (1, 2, 3)],
+                   ("flux", "fp64", False): ["mhd::k_flux<double, 1, %d, double, double, true>" % d for d in (1, 2, 3)],
+                   ("emf", "mixed", True): ["mhd::k_emf_march<float, 1, %d, 8, float, float>" % d for d in (1, 2, 3)],
+                   ("emf", "mixed", False): ["mhd::k_emf_march<float, 1, %d, 8, float, float>" % d for d in (1, 2, 3)],
+                   ("emf", "fp64", False): ["mhd::k_emf_march<double, 1, %d, 8, double, double>" % d
+                                            for d in (1, 2, 3)]}
+
+
+def measured_traffic(kcls, prec, sur, n):
+    """mean HBM bytes per launch of the kernel class, or None"""
+    if n != 256 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as fh:
+        ks = json.load(fh)["kernels"]
+    names = TRAFFIC_KERNELS.get((kcls, prec, bool(sur)))
+    if not names or not all(k in ks for k in names):
+        return None
+    return sum(ks[k]["fetch_bytes"] + ks[k]["write_bytes"] for k in names) / len(names)
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -234,6 +258,9 @@ def main():
     if args.only in ("fast", "both"):
         log("fast run (mixed precision%s) ..." % (" + learned reconstruction" if sur else ""))
         res["fast"] = run_gpu(g, st, n, "mixed", weights, args.warmup, args.steps)
+        if sur and args.only == "both":
+            log("mixed precision with PDM (no learned reconstruction) ...")
+            res["mixed_pdm"] = run_gpu(g, st, n, "mixed", None, args.warmup, args.steps)
     head = res.get("fast") or res["plain"]
     head_name = "fast" if "fast" in res else "plain"
     prec = "mixed" if head_name == "fast" else "fp64"
@@ -267,7 +294,7 @@ def main():
                    "grid": "sheared Cartesian, x = A x_comp, A = [[1,.5,0],[0,1,0],[0,0,1]]",
                    "parallelism": "single GPU, whole-domain block"},
         "roofline": {"bound": "hbm", "kernel": kcls, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kcls, prec, sur, n),
                      "avg_launch_ms": avg_s * 1e3, "alg_bytes_per_launch": alg},
     }
     if "plain" in res and "fast" in res:
@@ -277,6 +304,13 @@ def main():
         out["speedup_vs_plain_fp64"] = value / pv
         out["density_L1_vs_plain_fp64"] = l1_rel(res["fast"]["state"], res["plain"]["state"], g)
         out["max_divB"] = max_divb(res["fast"]["state"], g)
+    if "mixed_pdm" in res and "plain" in res:
+        r = res["mixed_pdm"]
+        out["mixed_fp32_pdm"] = {"value": n ** 3 * K / r["wall"], "ms_per_step": r["wall"] / K * 1e3,
+                                 "speedup_vs_plain_fp64": res["plain"]["wall"] / r["wall"],
+                                 "density_L1_vs_plain_fp64": l1_rel(r["state"], res["plain"]["state"], g),
+                                 "max_divB": max_divb(r["state"], g),
+                                 "kernel_ms_per_step": {k: v[1] / K for k, v in r["prof"].items() if v[0]}}
     out["kernel_ms_per_step"] = {k: v[1] / K for k, v in head["prof"].items() if v[0]}
     out["step_roofline"] = {"alg_bytes_per_zone_update": AB2_STATE_BYTES,
                             "achieved_GBs": value * AB2_STATE_BYTES / 1e9,

@@ -159,7 +159,7 @@ struct EmfGeom {
 template <typename R, int M, int K, int T, typename TI, typename TF>
 __global__ void __launch_bounds__(kBlock) k_emf_march(Dims d, Coef<R> c, EmfIn<TI> in, R ef, TF* __restrict__ e,
                                                      long nthreads) {
-    const long t = gtid();
+    const long t = gtid_xcd();
     if (t >= nthreads) return;
     const int ii = d.nx0 + (K != 1);
     const int no = K == 3 ? d.nz0 : (K == 1 ? d.ny0 + 1 : d.ny0);

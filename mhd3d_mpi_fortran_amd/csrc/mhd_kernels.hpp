@@ -42,6 +42,23 @@ __device__ __forceinline__ void unflat(long t, int a, int b, int& i, int& j, int
 
 __device__ __forceinline__ long gtid() { return (long)blockIdx.x * blockDim.x + threadIdx.x; }
 
+// XCD-aware block order.  The dispatcher hands consecutive workgroups to the
+// 8 XCDs in turn, and each XCD has its own L2: a stencil whose neighbouring
+// rows sit in neighbouring workgroups would fetch every row into several
+// L2s.  This bijection of [0, nblocks) splits every run of 8*kXcdRun logical
+// blocks into 8 sub-runs of kXcdRun, one per XCD: row neighbours share an
+// L2, while all XCDs still work inside one window (plane neighbours meet
+// in the shared last-level cache).
+constexpr unsigned kXcd = 8, kXcdRun = 16;
+__device__ __forceinline__ unsigned xcd_block() {
+    const unsigned b = blockIdx.x, n = gridDim.x;
+    const unsigned base = b / (kXcd * kXcdRun) * (kXcd * kXcdRun), r = b - base;
+    const unsigned m = n - base < kXcd * kXcdRun ? n - base : kXcd * kXcdRun;
+    const unsigned n8 = m / kXcd, rem = m % kXcd, x = r % kXcd, q = r / kXcd;
+    return base + x * n8 + (x < rem ? x : rem) + q;
+}
+__device__ __forceinline__ long gtid_xcd() { return (long)xcd_block() * blockDim.x + threadIdx.x; }
+
 template <typename T>
 struct Prim {  // cell-centred state read by the flux and EMF kernels
     const T *rho, *vx, *vy, *vz, *p, *bx, *by, *bz;
@@ -174,7 +191,7 @@ __device__ __forceinline__ void face_flux_met(const Coef<R>& c, const R* l, cons
 template <typename R, int M, int D, typename TI, typename TF, bool MET>
 __global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<TI> q, Normals<R> nm, TF* __restrict__ F,
                                                  long nface) {
-    const long t = gtid();
+    const long t = gtid_xcd();
     if (t >= nface) return;
     const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
     int i, j, k;

@@ -35,7 +35,7 @@ __global__ void __launch_bounds__(kBlock) k_update(Dims d, double gm1, const dou
                                                   Cells so, PrevRK pv, FluxSet<TF> F, const double* __restrict__ dx,
                                                   const double* __restrict__ dy, const double* __restrict__ dz,
                                                   long ncell) {
-    const long t = gtid();
+    const long t = gtid_xcd();
     if (t >= ncell) return;
     int i, j, k;
     unflat(t, d.nx0, d.ny0, i, j, k);
@@ -310,7 +310,7 @@ __global__ void __launch_bounds__(kBlock) k_bxyz(Dims d, double gam, double cfl,
                                                 const double* __restrict__ dx, const double* __restrict__ dy,
                                                 const double* __restrict__ dz, double* __restrict__ part, long ncell,
                                                 long nbi, long nbj, long nbk) {
-    const long t = gtid();
+    const long t = gtid_xcd();
     double dt = __builtin_huge_val();
     if (t < ncell) {
         int i, j, k;
