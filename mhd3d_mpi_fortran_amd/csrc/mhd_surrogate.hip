@@ -478,6 +478,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
                                                     unsigned long long* __restrict__ stats) {
     const int lane = threadIdx.x & 63;
     __shared__ NetW W;
+    __shared__ float stash[4][8][kBlock];
     load_net(W, wbf, bias);
     const ptrdiff_t st = D == 1 ? 1 : (D == 2 ? (ptrdiff_t)d.nx : (ptrdiff_t)d.nx * d.ny);
     const float* f[8] = {q.rho, q.vx, q.vy, q.vz, q.p, q.bx, q.by, q.bz};
@@ -518,8 +519,33 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
             A[v] = cc[v] + ss * lin4(x + 5 * v);
             G[v] = ss * gate;
         }
+        // the MLP only matters on faces that use the net and have a non-zero
+        // gate (all gates are zero on constant or linear windows, where
+        // y = lin exactly); MFMA needs the whole wave, so the test is per wave
         float y[NO];
-        mlp_core(W, x, lane, y);
+        const bool need = live && rough <= theta && rough > 0.f;
+        if (__ballot(need)) {
+            // A, G and the clamp range wait in LDS (each thread its own
+            // column) while the MLP holds the registers
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                stash[0][v][threadIdx.x] = A[v];
+                stash[1][v][threadIdx.x] = G[v];
+                stash[2][v][threadIdx.x] = lo[v];
+                stash[3][v][threadIdx.x] = hi[v];
+            }
+            mlp_core(W, x, lane, y);
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                A[v] = stash[0][v][threadIdx.x];
+                G[v] = stash[1][v][threadIdx.x];
+                lo[v] = stash[2][v][threadIdx.x];
+                hi[v] = stash[3][v][threadIdx.x];
+            }
+        } else {
+#pragma unroll
+            for (int o = 0; o < NO; ++o) y[o] = 0.f;
+        }
         float lv[8], rv[8];
 #pragma unroll
         for (int v = 0; v < 8; ++v) {
