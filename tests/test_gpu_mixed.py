@@ -1,0 +1,70 @@
+"""Mixed precision (fp32 stencils, binary64 accumulation and CT) against the
+binary64 path, field by field.
+
+The binary64 path is pinned to the reference (test_gpu_parity.py); here the
+error of the mixed path is stated for every output field: face fluxes of one
+state, and whole trajectories (PDM, AB2 and RK2; the sheared metric too).
+"""
+import numpy as np
+import pytest
+
+from mhd3d_mpi_fortran_amd import gpu, ics
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("rho", "vx", "vy", "vz", "p", "bi", "bj", "bk", "bx", "by", "bz")
+
+
+def _scale(st, k):
+    if k in ("vx", "vy", "vz"):
+        return max(np.abs(st[q]).max() for q in ("vx", "vy", "vz"))
+    if k in ("bi", "bj", "bk", "bx", "by", "bz"):
+        return max(np.abs(st[q]).max() for q in ("bi", "bj", "bk"))
+    return np.abs(st[k]).max()
+
+
+def test_mixed_flux_close_to_binary64():
+    g = ics.Grid(24, 20, 16, ng=4)
+    st = ics.initial_state(g, "orszag_tang", 51)
+    out = []
+    for prec in ("fp64", "mixed"):
+        s = gpu.Solver(gpu.make_params(24, 20, 16, reconop=1, stepop=2, bc="periodic", precision=prec), g)
+        s.upload(st)
+        s.initialize(1e-3)
+        out.append(s.flux_stress())
+        s.close()
+    for q in range(24):
+        a, b = out[1][q], out[0][q]
+        err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+        # fp32 stencils and moments; the stress components cancel (B^2/2 - B_n^2)
+        assert err <= 3e-5, (q, err)
+
+
+@pytest.mark.parametrize("stepop,sheared", [(1, False), (2, False), (1, True)])
+def test_mixed_trajectory_per_field(stepop, sheared):
+    n = 32
+    A = [[1., .5, 0.], [0., 1., 0.], [0., 0., 1.]] if sheared else None
+    g = ics.Grid(n, n, n, ng=4, A=A)
+    st = ics.initial_state(g, "blast", 7, radius=.2) if sheared else ics.initial_state(g, "orszag_tang", 52)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        s = gpu.Solver(gpu.make_params(n, n, n, reconop=1, stepop=stepop, bc="periodic", precision=prec), g)
+        s.upload(st)
+        tau = s.initialize(1e-3)
+        s.step(30, tau)
+        res[prec] = s.download()
+        assert s.check_state() == 0
+        s.close()
+    a, b = res["mixed"], res["fp64"]
+    for k in FIELDS:
+        face = {"bi": 1, "bj": 2, "bk": 3}.get(k, 0)
+        x, y = g.interior(a[k], face), g.interior(b[k], face)
+        sc = _scale(b, k)
+        # L1 relative to the field's own L1 (scalars) or to its vector's
+        # magnitude (components, which may be near zero everywhere)
+        den = np.abs(y).sum() if k in ("rho", "p") else sc * y.size
+        l1 = np.abs(x - y).sum() / den
+        mx = np.abs(x - y).max() / sc
+        print("stepop %d sheared %s %-3s  L1 %.2e  max/scale %.2e" % (stepop, sheared, k, l1, mx))
+        assert l1 <= 1e-5, (k, l1)
+        assert mx <= 5e-4, (k, mx)
