@@ -14,7 +14,14 @@
 !
 ! &gpu  bc_x, bc_y, bc_z   'even' (reference, default) or 'periodic'
 !       precision          'fp64' (default) or 'mixed'
-!       surrogate          0 (default) or 1
+!       surrogate          0 (default) or 1: learned face reconstruction
+!                          (precision 'mixed', reconop 1), weights from
+!       weights            a file of 2896 float32 (written by the Python
+!                          package's surrogate.save_weights)
+!       theta              its smoothness threshold (default 0.25)
+!       metric             9 reals, row-major A of an affine metric
+!                          x_phys = A x_comp (e.g. a sheared box); default
+!                          the identity, which keeps the reference's grid
 !       device             HIP device ordinal
 !       icfile             initial-state file (global owned arrays, float64,
 !                          column-major: rho vx vy vz p (nx0G,ny0G,nz0G),
@@ -36,14 +43,15 @@ program mhd3d_gpu
    real(8) :: cfl, pdmb, eps, alpha, beta
    character(len=16) :: bc_x, bc_y, bc_z, precision
    integer :: surrogate, device
-   character(len=512) :: icfile, outdir
+   character(len=512) :: icfile, outdir, weights
+   real(8) :: theta, metric(9)
 
    namelist /dom/ xmin, xmax, ymin, ymax, zmin, zmax
    namelist /grid/ nx0G, ny0G, nz0G, xprocs, yprocs, zprocs
    namelist /comset/ nord, reconop, stepop, outop
    namelist /list/ nts, tau, nframes, nrst, nfdump
    namelist /free/ cfl, pdmb, eps, alpha, beta
-   namelist /gpu/ bc_x, bc_y, bc_z, precision, surrogate, device, icfile, outdir
+   namelist /gpu/ bc_x, bc_y, bc_z, precision, surrogate, device, icfile, outdir, weights, theta, metric
 
    real(8), allocatable :: rho(:,:,:), vx(:,:,:), vy(:,:,:), vz(:,:,:), p(:,:,:)
    real(8), allocatable :: bx(:,:,:), by(:,:,:), bz(:,:,:)
@@ -62,7 +70,8 @@ program mhd3d_gpu
 
    ! defaults of the &gpu group
    bc_x = 'even'; bc_y = 'even'; bc_z = 'even'; precision = 'fp64'
-   surrogate = 0; device = 0; icfile = ''; outdir = ''
+   surrogate = 0; device = 0; icfile = ''; outdir = ''; weights = ''; theta = 0.25d0
+   metric = [1.d0, 0.d0, 0.d0, 0.d0, 1.d0, 0.d0, 0.d0, 0.d0, 1.d0]
    xprocs = 1; yprocs = 1; zprocs = 1; outop = 0; nrst = 0; nfdump = 0
 
    print*, 'reading input file ', trim(deck)
@@ -113,6 +122,9 @@ program mhd3d_gpu
 
    call mhd_check(mhd_create(ctx, prm), 'mhd_create')
    call mhd_check(mhd_set_geometry(ctx, dx, dy, dz), 'mhd_set_geometry')
+   if (any(metric /= [1.d0, 0.d0, 0.d0, 0.d0, 1.d0, 0.d0, 0.d0, 0.d0, 1.d0])) &
+      call mhd_check(mhd_set_metric(ctx, metric), 'mhd_set_metric')
+   if (surrogate /= 0) call load_weights(trim(weights))
    call mhd_check(mhd_upload_state(ctx, rho, vx, vy, vz, p, bi, bj, bk), 'mhd_upload_state')
    call mhd_check(mhd_initialize(ctx, tau), 'mhd_initialize')
    print*, 'initialization finished!'
@@ -153,6 +165,19 @@ program mhd3d_gpu
    call mhd_destroy(ctx)
 
 contains
+
+   ! trained weights of the learned reconstruction: MHD_SUR_NPARAM float32
+   subroutine load_weights(path)
+      character(len=*), intent(in) :: path
+      real(c_float) :: w(MHD_SUR_NPARAM)
+      integer :: uw
+      if (len(path) == 0) stop 'surrogate = 1 needs weights = <file>'
+      open(newunit=uw, file=path, access='stream', form='unformatted', status='old')
+      read(uw) w
+      close(uw)
+      call mhd_check(mhd_surrogate_set_weights(ctx, w, MHD_SUR_NPARAM), 'mhd_surrogate_set_weights')
+      call mhd_check(mhd_surrogate_configure(ctx, theta, 0_c_int), 'mhd_surrogate_configure')
+   end subroutine load_weights
 
    integer(c_int) function bc_code(s)
       character(len=*), intent(in) :: s

@@ -109,7 +109,33 @@ module mhd_gpu
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
       end function
+      ! affine metric x_phys = A x_comp, A row-major (no reference counterpart)
+      integer(c_int) function mhd_set_metric(ctx, a) bind(C, name='mhd_set_metric')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: a(9)
+      end function
+      ! learned reconstruction (no reference counterpart)
+      integer(c_int) function mhd_surrogate_set_weights(ctx, w, n) bind(C, name='mhd_surrogate_set_weights')
+         import :: c_int, c_ptr, c_float
+         type(c_ptr), value :: ctx
+         real(c_float), intent(in) :: w(*)
+         integer(c_int), value :: n
+      end function
+      integer(c_int) function mhd_surrogate_configure(ctx, theta, count) bind(C, name='mhd_surrogate_configure')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), value :: theta
+         integer(c_int), value :: count
+      end function
+      integer(c_int) function mhd_surrogate_stats(ctx, faces, fallbacks) bind(C, name='mhd_surrogate_stats')
+         import :: c_int, c_ptr, c_long
+         type(c_ptr), value :: ctx
+         integer(c_long), intent(out) :: faces, fallbacks
+      end function
    end interface
+
+   integer(c_int), parameter :: MHD_SUR_NPARAM = 2896
 
 contains
 

@@ -62,6 +62,18 @@ def load_samples(solver, samples):
     gpu._ck(gpu.lib().mhd_surrogate_load_samples(solver.h, _pf(s), s.shape[0]), "mhd_surrogate_load_samples")
 
 
+def save_weights(w, path):
+    """raw float32 weights (the Fortran driver's &gpu weights= file)"""
+    np.ascontiguousarray(w, dtype="<f4").tofile(path)
+
+
+def load_weights(path):
+    w = np.fromfile(path, dtype="<f4")
+    if w.size != NPARAM:
+        raise ValueError("%s holds %d values, expected %d" % (path, w.size, NPARAM))
+    return w
+
+
 def train(solver, epochs=4, batch=1024, lr=5e-3, seed=0):
     loss = C.c_double(0.)
     gpu._ck(gpu.lib().mhd_surrogate_train(solver.h, int(epochs), int(batch), float(lr), int(seed), C.byref(loss)),
