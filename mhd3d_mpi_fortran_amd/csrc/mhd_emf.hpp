@@ -66,19 +66,21 @@ struct VavWin {
     }
 };
 
-// corner velocity of one edge from its window (mhd3d3.f90:1858-2116)
+// PDM: the d1-interpolation of footprint column m2 (offset m2-4 along d2)
+template <typename R, bool MD1>
+__device__ __forceinline__ R pdm_col(const VavWin<R, PDM, MD1>& W, int m2) {
+    R s[8];
+#pragma unroll
+    for (int m1 = 0; m1 < 8; ++m1) s[m1] = W.cv(m1 - 4, m2 - 4);
+    return interp8(s);
+}
+
+// corner velocity of one edge from its window (mhd3d3.f90:1858-2116); for
+// PDM, g holds the 8 column interpolations (interp2_pdm's first sweep)
 template <typename R, int M, bool MD1>
-__device__ __forceinline__ R vav_win(const VavWin<R, M, MD1>& W, const Coef<R>& c) {
+__device__ __forceinline__ R vav_win(const VavWin<R, M, MD1>& W, const Coef<R>& c, const R* g = nullptr) {
     R v00_0, v10_0, v01_0, v11_0, v00_1, v01_1, v10_1, v11_1;
     if constexpr (M == PDM) {
-        R g[8];
-#pragma unroll
-        for (int m2 = 0; m2 < 8; ++m2) {
-            R s[8];
-#pragma unroll
-            for (int m1 = 0; m1 < 8; ++m1) s[m1] = W.cv(m1 - 4, m2 - 4);
-            g[m2] = interp8(s);
-        }
         const R vi = interp8(g);
         pdm2(W.cv(-2, -1), W.cv(-1, -1), W.cv(0, -1), W.cv(1, -1), vi, c.pdmb, v00_0, v10_0);
         pdm2(W.cv(-1, -2), W.cv(-1, -1), W.cv(-1, 0), W.cv(-1, 1), vi, c.pdmb, v00_1, v01_1);
@@ -124,19 +126,38 @@ __device__ __forceinline__ R vav_win(const VavWin<R, M, MD1>& W, const Coef<R>& 
 }
 
 // corner velocities of `cnt` consecutive edges along the march axis
+// When the march runs along d2 (K=1, K=3), a step shifts the footprint by
+// one column, so seven of PDM's eight column interpolations carry over: they
+// ride along in a register window and only the new column is interpolated
+// (the same interp8 calls on the same values: bit-identical).
 template <typename R, int M, bool MD1, int T, typename TI>
 __device__ __forceinline__ void vav_march(const TI* __restrict__ lv, ptrdiff_t b, ptrdiff_t s1, ptrdiff_t s2,
                                           int cnt, const Coef<R>& c, R (&out)[T]) {
     const ptrdiff_t sm = MD1 ? s1 : s2, so = MD1 ? s2 : s1;
     VavWin<R, M, MD1> W;
     W.load(lv, b, sm, so);
+    R g[8];
+    if constexpr (M == PDM) {
+#pragma unroll
+        for (int m2 = 0; m2 < 8; ++m2) g[m2] = pdm_col<R, MD1>(W, m2);
+    }
 #pragma unroll
     for (int s = 0; s < T; ++s) {
         if (s < cnt) {
-            out[s] = vav_win<R, M, MD1>(W, c);
+            out[s] = vav_win<R, M, MD1>(W, c, g);
             if (s + 1 < cnt) {
                 b += sm;
                 W.shift(lv, b, sm, so);
+                if constexpr (M == PDM) {
+                    if constexpr (MD1) {
+#pragma unroll
+                        for (int m2 = 0; m2 < 8; ++m2) g[m2] = pdm_col<R, MD1>(W, m2);
+                    } else {
+#pragma unroll
+                        for (int m2 = 0; m2 < 7; ++m2) g[m2] = g[m2 + 1];
+                        g[7] = pdm_col<R, MD1>(W, 7);
+                    }
+                }
             }
         }
     }
