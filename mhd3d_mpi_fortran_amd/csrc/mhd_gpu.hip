@@ -181,6 +181,7 @@ void mhd_destroy(mhd_ctx* c) {
 }
 
 static int upload_metric_spacing(mhd_ctx* c);
+static void ensure_cellb(mhd_ctx* c);
 
 int mhd_set_geometry(mhd_ctx* c, const double* dx, const double* dy, const double* dz) {
     if (!c || !dx || !dy || !dz) return MHD_ERR_PARAM;
@@ -296,6 +297,7 @@ int mhd_download_state(mhd_ctx* c, double* rho, double* vx, double* vy, double* 
                        double* bj, double* bk, double* bx, double* by, double* bz) {
     if (!c) return MHD_ERR_PARAM;
     hipSetDevice(c->p.device);
+    if (bx || by || bz) ensure_cellb(c);
     const size_t cb = c->ncell * sizeof(double);
     void* dst[11] = {rho, vx, vy, vz, p, bi, bj, bk, bx, by, bz};
     const void* src[11] = {c->rho, c->vx, c->vy, c->vz, c->pr, c->bi, c->bj, c->bk, c->bx, c->by, c->bz};
@@ -381,7 +383,7 @@ static void do_fill(mhd_ctx* c) {
 
 // get_bxyz on the current state, plus (optionally) vf, the fp32 stencil
 // copies and the CFL partials of the next get_dt
-static void do_bxyz(mhd_ctx* c, bool want_vf, bool want_shadow, bool want_dt) {
+static void do_bxyz(mhd_ctx* c, bool want_vf, bool want_shadow, bool want_dt, bool store_cellb = true) {
     Shadow sh{};
     if (want_shadow && mixed(c))
         sh = Shadow{c->sh[0], c->sh[1], c->sh[2], c->sh[3], c->sh[4],  c->sh[5],
@@ -390,10 +392,11 @@ static void do_bxyz(mhd_ctx* c, bool want_vf, bool want_shadow, bool want_dt) {
     // fp32 with the stencil copies
     const BMet m = bmet(c, want_vf && !mixed(c), want_shadow && mixed(c));
     launch(c, MHD_K_BXYZ, blocks(c->nmax), dim3(kBlock), k_bxyz, c->d, c->p.gam, c->p.cfl,
-           CFaces{c->bi, c->bj, c->bk}, CCells{c->rho, c->vx, c->vy, c->vz, c->pr}, c->bx, c->by, c->bz,
-           want_vf ? c->vf : (double*)nullptr, sh, m, sdx(c), sdy(c), sdz(c),
-           want_dt ? c->dtpart : (double*)nullptr, c->ncell, c->nbi, c->nbj, c->nbk);
+           CFaces{c->bi, c->bj, c->bk}, CCells{c->rho, c->vx, c->vy, c->vz, c->pr},
+           store_cellb ? c->bx : (double*)nullptr, c->by, c->bz, want_vf ? c->vf : (double*)nullptr, sh, m,
+           sdx(c), sdy(c), sdz(c), want_dt ? c->dtpart : (double*)nullptr, c->ncell, c->nbi, c->nbj, c->nbk);
     c->dt_fresh = want_dt;
+    c->cellb_stale = !store_cellb;
 }
 
 // get_dt on the current state; the minimum lands in scal[SC_DTCFL] and, when
@@ -481,6 +484,7 @@ int mhd_get_bxyz(mhd_ctx* c) {
 int mhd_get_dt(mhd_ctx* c, double* tau_cfl) {
     if (!c || !c->have_geometry) return MHD_ERR_PARAM;
     hipSetDevice(c->p.device);
+    if (!c->dt_fresh) ensure_cellb(c);
     do_dt(c, false);
     double h = 0.;
     if (hipMemcpyAsync(&h, c->scal + SC_DTCFL, sizeof(double), hipMemcpyDeviceToHost, c->s) != hipSuccess)
@@ -520,9 +524,15 @@ int mhd_eijk_lfm(mhd_ctx* c) {
 // vf and fp32 copies are kept only where the next step reads them: RK reads
 // the current state (vf in binary64, the copies in mixed precision), AB
 // reads the extrapolated state written by the predictor
-static void end_of_stage(mhd_ctx* c) {
+static void end_of_stage(mhd_ctx* c, bool in_step = false) {
     const bool rk = c->p.stepop != 1;
-    do_bxyz(c, rk && !mixed(c), rk && mixed(c), true);
+    // mixed Adams-Bashforth steps take the extrapolated cell B from the faces
+    // (k_predict_ab_mixed), so the step itself leaves cell B implicit
+    do_bxyz(c, rk && !mixed(c), rk && mixed(c), true, !(in_step && !rk && mixed(c)));
+}
+// cell B of the current state where a caller reads it
+static void ensure_cellb(mhd_ctx* c) {
+    if (c->cellb_stale) do_bxyz(c, false, false, false);
 }
 
 int mhd_initialize(mhd_ctx* c, double* tau) {
@@ -588,7 +598,8 @@ static void step_ab(mhd_ctx* c) {
         pj.on = c->met.on ? 1 : 0;
         for (int q = 0; q < 9; ++q) pj.n[q] = c->met.n[q / 3][q % 3];
         for (int d = 0; d < 3; ++d) pj.w[d] = c->w32[d];
-        launch(c, MHD_K_PREV, blocks(c->nmax), dim3(kBlock), k_predict_ab<float>, s,
+        const FaceB fb{c->bi, c->bj, c->bk, c->xp[8], c->xp[9], c->xp[10]};
+        launch(c, MHD_K_PREV, blocks(c->nmax), dim3(kBlock), k_predict_ab_mixed, c->d, s, fb, bmet(c, false, false),
                (const double*)(c->scal + SC_TAU), c->sh[8], pj, nc, nbi, nbj, nbk);
         launch(c, MHD_K_PREV, dim3(1), dim3(64), k_tau0_from_tau, c->scal + SC_TAU);
         do_ops32(c);
@@ -617,7 +628,7 @@ static void step_ab(mhd_ctx* c) {
         c->xp[a] = cur;
     }
     do_fill(c);
-    end_of_stage(c);
+    end_of_stage(c, true);
 }
 
 int mhd_step(mhd_ctx* c, int nsteps, double* tau, double* tau_cfl) {
@@ -734,6 +745,11 @@ int mhd_synchronize(mhd_ctx* c) {
 
 void* mhd_device_pointer(mhd_ctx* c, int which) {
     if (!c) return nullptr;
+    if (which >= 5 && which <= 7 && c->cellb_stale) {  // cell B: materialise it first
+        hipSetDevice(c->p.device);
+        ensure_cellb(c);
+        if (hipStreamSynchronize(c->s) != hipSuccess) return nullptr;
+    }
     void* t[11] = {c->rho, c->vx, c->vy, c->vz, c->pr, c->bx, c->by, c->bz, c->bi, c->bj, c->bk};
     return which >= 0 && which < 11 ? t[which] : nullptr;
 }

@@ -331,9 +331,11 @@ __global__ void __launch_bounds__(kBlock) k_bxyz(Dims d, double gam, double cfl,
                 if (met.ws[a]) met.ws[a][t] = float(wn);
             }
         }
-        bx[t] = x;
-        by[t] = y;
-        bz[t] = z;
+        if (bx) {  // null: the caller keeps cell B implicit (mixed Adams-Bashforth)
+            bx[t] = x;
+            by[t] = y;
+            bz[t] = z;
+        }
         if (vf || sh.rho) {
             const double f = sqrt(u * u + v * v + w * w) + sqrt((x * x + y * y + z * z) / r);
             if (vf) vf[t] = f;
@@ -493,6 +495,65 @@ __global__ void __launch_bounds__(kBlock) k_predict_ab(AB11<TH> s, const double*
         if (t < nf[a]) {
             const double x = s.x[8 + a][t], xp = s.xp[8 + a][t];
             s.xh[8 + a][t] = TH(x + c * (x - xp));
+        }
+}
+
+// Mixed-precision Adams-Bashforth: the same extrapolation, with the cell B
+// of the extrapolated state taken from its extrapolated faces
+// (M (face averages), as get_bxyz does) instead of from the cell B of the
+// current and previous states -- equal in exact arithmetic, and the cell B
+// arrays need not be stored each step.  Cell threads read their faces'
+// neighbours (f+1 along each axis), which the neighbouring threads also
+// read: the extra taps hit the caches.
+struct FaceB {
+    const double *bi, *bj, *bk, *bip, *bjp, *bkp;
+};
+
+__global__ void __launch_bounds__(kBlock) k_predict_ab_mixed(Dims d, AB11<float> s, FaceB fb, BMet met,
+                                                            const double* __restrict__ tau_p,
+                                                            float* __restrict__ vfh, Proj<float> pj, long ncell,
+                                                            long nbi, long nbj, long nbk) {
+    const long t = gtid();
+    const double c = tau_p[0] / tau_p[1] / 2.;
+    if (t < ncell) {
+        double h[8];
+#pragma unroll
+        for (int a = 0; a < 5; ++a) {
+            const double x = s.x[a][t], xp = s.xp[a][t];
+            h[a] = x + c * (x - xp);
+            s.xh[a][t] = float(h[a]);
+        }
+        int i, j, k;
+        unflat(t, d.nx, d.ny, i, j, k);
+        auto ext = [&](const double* x, const double* xp, ptrdiff_t f) { return x[f] + c * (x[f] - xp[f]); };
+        const ptrdiff_t fi = ix(d.nx + 1, d.ny, i, j, k), fj = ix(d.nx, d.ny + 1, i, j, k);
+        const double ax = .5 * (ext(fb.bi, fb.bip, fi + 1) + ext(fb.bi, fb.bip, fi));
+        const double ay = .5 * (ext(fb.bj, fb.bjp, fj + d.nx) + ext(fb.bj, fb.bjp, fj));
+        const double az = .5 * (ext(fb.bk, fb.bkp, t + (ptrdiff_t)d.nx * d.ny) + ext(fb.bk, fb.bkp, t));
+        if (met.on) {
+            h[5] = met.M[0] * ax + met.M[1] * ay + met.M[2] * az;
+            h[6] = met.M[3] * ax + met.M[4] * ay + met.M[5] * az;
+            h[7] = met.M[6] * ax + met.M[7] * ay + met.M[8] * az;
+        } else {
+            h[5] = ax;
+            h[6] = ay;
+            h[7] = az;
+        }
+#pragma unroll
+        for (int a = 5; a < 8; ++a) s.xh[a][t] = float(h[a]);
+        vfh[t] = float(sqrt(h[1] * h[1] + h[2] * h[2] + h[3] * h[3]) + sqrt((h[5] * h[5] + h[6] * h[6] + h[7] * h[7]) / h[0]));
+        if (pj.on) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                pj.w[a][t] = float(pj.n[3 * a] * h[1] + pj.n[3 * a + 1] * h[2] + pj.n[3 * a + 2] * h[3]);
+        }
+    }
+    const long nf[3] = {nbi, nbj, nbk};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (t < nf[a]) {
+            const double x = s.x[8 + a][t], xp = s.xp[8 + a][t];
+            s.xh[8 + a][t] = float(x + c * (x - xp));
         }
 }
 
