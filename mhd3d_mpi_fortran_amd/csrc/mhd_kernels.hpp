@@ -49,11 +49,14 @@ __device__ __forceinline__ long gtid() { return (long)blockIdx.x * blockDim.x + 
 // blocks into 8 sub-runs of kXcdRun, one per XCD: row neighbours share an
 // L2, while all XCDs still work inside one window (plane neighbours meet
 // in the shared last-level cache).
+// RUN is in workgroups of 256 points; a kernel whose workgroups cover G x 256
+// points passes 16 / G, so a run always spans 4096 points.
 constexpr unsigned kXcd = 8, kXcdRun = 16;
+template <unsigned RUN = kXcdRun>
 __device__ __forceinline__ unsigned xcd_block() {
     const unsigned b = blockIdx.x, n = gridDim.x;
-    const unsigned base = b / (kXcd * kXcdRun) * (kXcd * kXcdRun), r = b - base;
-    const unsigned m = n - base < kXcd * kXcdRun ? n - base : kXcd * kXcdRun;
+    const unsigned base = b / (kXcd * RUN) * (kXcd * RUN), r = b - base;
+    const unsigned m = n - base < kXcd * RUN ? n - base : kXcd * RUN;
     const unsigned n8 = m / kXcd, rem = m % kXcd, x = r % kXcd, q = r / kXcd;
     return base + x * n8 + (x < rem ? x : rem) + q;
 }

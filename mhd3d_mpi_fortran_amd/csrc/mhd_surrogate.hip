@@ -484,7 +484,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
     const float* f[8] = {q.rho, q.vx, q.vy, q.vz, q.p, q.bx, q.by, q.bz};
     const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
     for (int grp = 0; grp < SUR_GROUPS; ++grp) {
-        const long t = ((long)xcd_block() * SUR_GROUPS + grp) * blockDim.x + threadIdx.x;
+        const long t = ((long)xcd_block<kXcdRun / SUR_GROUPS>() * SUR_GROUPS + grp) * blockDim.x + threadIdx.x;
         const bool live = t < nface;
         if (__ballot(live) == 0ull) break;  // wave-uniform: later groups are past the end too
         int i = 1, j = 1, k = 1;
