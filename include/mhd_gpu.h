@@ -80,6 +80,32 @@ int mhd_set_geometry(mhd_ctx *ctx, const double *dx, const double *dy, const dou
  * switches the context to the metric kernels (also for A = I). */
 int mhd_set_metric(mhd_ctx *ctx, const double *A);
 
+/* general curvilinear metric (no counterpart in the reference, whose grid is
+ * the tensor product of init_geom; for grids such as a spherical shell
+ * given by their node coordinates): X, Y, Z hold the physical coordinates of the
+ * (nx+1)(ny+1)(nz+1) nodes, column-major, node (i,j,k) the lower corner of
+ * cell (i,j,k), ghost cells included.  Per-face unit normals and areas,
+ * per-cell volumes, cell-B coefficients and CFL spacings, and per-edge EMF
+ * factors are computed once on the host and kept on the device.  Same
+ * variable meaning as mhd_set_metric; an affine grid given by its nodes
+ * reproduces mhd_set_metric's run.  MHD_ERR_PARAM for a degenerate or
+ * inverted cell, face or edge.  Requires mhd_set_geometry first (the index
+ * space); the spacings given there are not used afterwards. */
+int mhd_set_metric_nodes(mhd_ctx *ctx, const double *X, const double *Y, const double *Z);
+
+/* the general metric's arrays as mhd_set_metric_nodes builds them, on the
+ * host (no device call; no counterpart in the reference).  out == NULL:
+ * returns the number of doubles needed.  Otherwise fills out, in order: for
+ * each face family f = i, j, k the unit normals (3 components, each over the
+ * ghost-inclusive face layout), the areas and their reciprocals; for each
+ * edge direction the EMF factor per owned edge; then per cell (ghost-
+ * inclusive) 1/V, the 18 cell-B coefficients (face lo/hi of i, j, k times
+ * x, y, z), the 9 cell normals and the 3 CFL factors, each component a
+ * contiguous array.  Returns that count, or -MHD_ERR_PARAM for a bad size
+ * or a degenerate or inverted cell, face or edge. */
+long mhd_metric_nodes_host(int nx0, int ny0, int nz0, int ng, const double *X, const double *Y, const double *Z,
+                           double *out);
+
 /* copy the primitive state in / out (full ghost-inclusive arrays).  Any of
  * the download pointers may be NULL. */
 int mhd_upload_state(mhd_ctx *ctx, const double *rho, const double *vx, const double *vy, const double *vz,

@@ -178,7 +178,8 @@ struct EmfGeom {
 
 // threads = ii * no * ceil(nm / T); lanes run along x
 template <typename R, int M, int K, int T, typename TI, typename TF>
-__global__ void __launch_bounds__(kBlock) k_emf_march(Dims d, Coef<R> c, EmfIn<TI> in, R ef, TF* __restrict__ e,
+__global__ void __launch_bounds__(kBlock) k_emf_march(Dims d, Coef<R> c, EmfIn<TI> in, R ef,
+                                                     const double* __restrict__ efa, TF* __restrict__ e,
                                                      long nthreads) {
     const long t = gtid_xcd();
     if (t >= nthreads) return;
@@ -254,9 +255,10 @@ __global__ void __launch_bounds__(kBlock) k_emf_march(Dims d, Coef<R> c, EmfIn<T
             else
                 eta = R(.25) * (R(in.vf[bc - sx - sy]) + R(in.vf[bc - sy]) + R(in.vf[bc]) + R(in.vf[bc - sx]));
             dum = dum + eta * (b1l + b2r - b1r - b2l);
-            // ef = |C_a||C_b|/det A of a metric (E . a_k per unit computational
-            // length); exactly 1 on the reference's tensor-product grid
-            e[eo] = TF(dum * ef);
+            // ef = |C_a||C_b|/det A of an affine metric (E . a_k per unit
+            // computational length), exactly 1 on the reference's grid; a
+            // general metric gives E . dl per edge (efa)
+            e[eo] = TF(dum * (efa ? R(efa[eo]) : ef));
             base1 += mv1;
             base2 += mv2;
             bc += smarch;

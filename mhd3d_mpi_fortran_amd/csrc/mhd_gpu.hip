@@ -175,6 +175,7 @@ void mhd_destroy(mhd_ctx* c) {
     }
     surrogate_free(c);
     if (c->metbuf) hipFree(c->metbuf);
+    if (c->gmetbuf) hipFree(c->gmetbuf);
     if (c->arena) hipFree(c->arena);
     if (c->s) hipStreamDestroy(c->s);
     delete c;
@@ -215,6 +216,34 @@ static int upload_metric_spacing(mhd_ctx* c) {
     return MHD_OK;
 }
 
+// face-normal velocity arrays and the metric spacing (once per context)
+static int alloc_metbuf(mhd_ctx* c) {
+    if (c->metbuf) return MHD_OK;
+    const size_t cb = c->ncell * sizeof(double), fb = c->ncell * sizeof(float);
+    const size_t sb = (c->d.nx + c->d.ny + c->d.nz) * sizeof(double);
+    const bool ab = c->p.stepop == 1, mx = c->p.precision == MHD_MIXED;
+    const size_t total = sb + 3 * cb + (ab && !mx ? 3 * cb : 0) + (mx ? 3 * fb : 0) + 8 * 256;
+    if (hipMalloc(&c->metbuf, total) != hipSuccess) {
+        c->metbuf = nullptr;
+        return MHD_ERR_HIP;
+    }
+    char* q = c->metbuf;
+    auto take = [&](size_t n) {
+        char* r = q;
+        q += (n + 255) & ~size_t(255);
+        return r;
+    };
+    c->dxm = (double*)take(c->d.nx * sizeof(double));
+    c->dym = (double*)take(c->d.ny * sizeof(double));
+    c->dzm = (double*)take(c->d.nz * sizeof(double));
+    for (int d = 0; d < 3; ++d) c->w64[d] = (double*)take(cb);
+    if (ab && !mx)
+        for (int d = 0; d < 3; ++d) c->wh64[d] = (double*)take(cb);
+    if (mx)
+        for (int d = 0; d < 3; ++d) c->w32[d] = (float*)take(fb);
+    return MHD_OK;
+}
+
 int mhd_set_metric(mhd_ctx* c, const double* A) {
     if (!c || !A) return MHD_ERR_PARAM;
     hipSetDevice(c->p.device);
@@ -233,6 +262,7 @@ int mhd_set_metric(mhd_ctx* c, const double* A) {
     if (!(J > 0.) || !std::isfinite(J)) return MHD_ERR_PARAM;
     Metric m;
     m.on = true;
+    m.mode = 1;
     for (int q = 0; q < 9; ++q) m.A[q] = A[q];
     double len[3];
     for (int d = 0; d < 3; ++d) {
@@ -244,31 +274,233 @@ int mhd_set_metric(mhd_ctx* c, const double* A) {
     for (int d = 0; d < 3; ++d) m.ef[d] = len[(d + 1) % 3] * len[(d + 2) % 3] / J;
     for (int r = 0; r < 3; ++r)
         for (int d = 0; d < 3; ++d) m.M[3 * r + d] = A[3 * r + d] * m.g[d];
-    if (!c->metbuf) {
-        const size_t cb = c->ncell * sizeof(double), fb = c->ncell * sizeof(float);
-        const size_t sb = (c->d.nx + c->d.ny + c->d.nz) * sizeof(double);
-        const bool ab = c->p.stepop == 1, mx = c->p.precision == MHD_MIXED;
-        const size_t total = sb + 3 * cb + (ab && !mx ? 3 * cb : 0) + (mx ? 3 * fb : 0) + 1024;
-        if (hipMalloc(&c->metbuf, total) != hipSuccess) {
-            c->metbuf = nullptr;
-            return MHD_ERR_HIP;
-        }
-        char* q = c->metbuf;
-        auto take = [&](size_t n) {
-            char* r = q;
-            q += (n + 255) & ~size_t(255);
-            return r;
-        };
-        c->dxm = (double*)take(c->d.nx * sizeof(double));
-        c->dym = (double*)take(c->d.ny * sizeof(double));
-        c->dzm = (double*)take(c->d.nz * sizeof(double));
-        for (int d = 0; d < 3; ++d) c->w64[d] = (double*)take(cb);
-        if (ab && !mx)
-            for (int d = 0; d < 3; ++d) c->wh64[d] = (double*)take(cb);
-        if (mx)
-            for (int d = 0; d < 3; ++d) c->w32[d] = (float*)take(fb);
-    }
+    if (alloc_metbuf(c) != MHD_OK) return MHD_ERR_HIP;
     c->met = m;
+    c->dt_fresh = false;
+    if (upload_metric_spacing(c) != MHD_OK) return MHD_ERR_HIP;
+    return MHD_OK;
+}
+
+// general metric from the node coordinates X, Y, Z ((nx+1)(ny+1)(nz+1) each,
+// column-major, node (i,j,k) = lower corner of cell (i,j,k), ghost cells
+// included).  Face area vectors S = (d1 x d2)/2 of the face diagonals
+// (oriented along increasing index), volumes by the divergence theorem with
+// face centroids, cell B coefficients from B_c = (1/V) sum_f (B.S_out)(x_f - x_c),
+// and for each owned edge the factor |S1||S2|/V of its local basis (edge
+// vector dl and the averaged transverse node differences a1, a2), which
+// turns the face-normal velocities and fields into E . dl exactly for a
+// uniform state on an affine grid.
+long mhd_metric_nodes_host(int nx0, int ny0, int nz0, int ng, const double* X, const double* Y, const double* Z,
+                           double* out) {
+    if (nx0 < 1 || ny0 < 1 || nz0 < 1 || ng < 1) return -MHD_ERR_PARAM;
+    const int nx = nx0 + 2 * ng, ny = ny0 + 2 * ng, nz = nz0 + 2 * ng;
+    const long ncell = (long)nx * ny * nz;
+    const long nb[3] = {(long)(nx + 1) * ny * nz, (long)nx * (ny + 1) * nz, (long)nx * ny * (nz + 1)};
+    const long ne[3] = {(long)nx0 * (ny0 + 1) * (nz0 + 1), (long)(nx0 + 1) * ny0 * (nz0 + 1),
+                        (long)(nx0 + 1) * (ny0 + 1) * nz0};
+    long total = 31 * ncell;  // 1/V, 18 cell-B coefficients, 9 normals, 3 CFL factors
+    for (int f = 0; f < 3; ++f) total += 5 * nb[f] + ne[f];
+    if (!out) return total;
+    if (!X || !Y || !Z) return -MHD_ERR_PARAM;
+    const long nn1 = nx + 1, nn2 = ny + 1;
+    auto node = [&](int i, int j, int k, double* p) {  // 1-based node
+        const long q = (long)(i - 1) + nn1 * ((long)(j - 1) + nn2 * (long)(k - 1));
+        p[0] = X[q];
+        p[1] = Y[q];
+        p[2] = Z[q];
+    };
+    auto sub = [](const double* a, const double* b, double* r) {
+        for (int q = 0; q < 3; ++q) r[q] = a[q] - b[q];
+    };
+    auto cross = [](const double* a, const double* b, double* r) {
+        r[0] = a[1] * b[2] - a[2] * b[1];
+        r[1] = a[2] * b[0] - a[0] * b[2];
+        r[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    auto dot = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    auto norm = [&](const double* a) { return std::sqrt(dot(a, a)); };
+    // face geometry: area vector and centroid of face (i,j,k) of family f (1-based, ghost-inclusive)
+    auto face = [&](int f, int i, int j, int k, double* S, double* xc) {
+        double p00[3], p10[3], p01[3], p11[3];  // corners in the face's (a, b) directions
+        if (f == 0) {       // a = y, b = z
+            node(i, j, k, p00); node(i, j + 1, k, p10); node(i, j, k + 1, p01); node(i, j + 1, k + 1, p11);
+        } else if (f == 1) {  // a = z, b = x
+            node(i, j, k, p00); node(i, j, k + 1, p10); node(i + 1, j, k, p01); node(i + 1, j, k + 1, p11);
+        } else {            // a = x, b = y
+            node(i, j, k, p00); node(i + 1, j, k, p10); node(i, j + 1, k, p01); node(i + 1, j + 1, k, p11);
+        }
+        double d1[3], d2[3];
+        sub(p11, p00, d1);
+        sub(p01, p10, d2);
+        cross(d1, d2, S);
+        for (int q = 0; q < 3; ++q) {
+            S[q] *= .5;
+            xc[q] = .25 * (p00[q] + p10[q] + p01[q] + p11[q]);
+        }
+    };
+    // host arrays
+    std::vector<double> nf[3], af[3], iaf[3], ef[3];
+    for (int f = 0; f < 3; ++f) {
+        nf[f].assign(3 * nb[f], 0.);
+        af[f].assign(nb[f], 0.);
+        iaf[f].assign(nb[f], 0.);
+        ef[f].assign(ne[f], 0.);
+    }
+    std::vector<double> ivol(ncell), cm(18 * ncell), nc(9 * ncell), ih(3 * ncell);
+    const int fe[3][3] = {{nx + 1, ny, nz}, {nx, ny + 1, nz}, {nx, ny, nz + 1}};
+    for (int f = 0; f < 3; ++f)
+        for (int k = 1; k <= fe[f][2]; ++k)
+            for (int j = 1; j <= fe[f][1]; ++j)
+                for (int i = 1; i <= fe[f][0]; ++i) {
+                    double S[3], xc[3];
+                    face(f, i, j, k, S, xc);
+                    const long q = (long)(i - 1) + (long)fe[f][0] * ((long)(j - 1) + (long)fe[f][1] * (k - 1));
+                    const double a = norm(S);
+                    if (!(a > 0.)) return -MHD_ERR_PARAM;
+                    af[f][q] = a;
+                    iaf[f][q] = 1. / a;
+                    for (int r = 0; r < 3; ++r) nf[f][r * nb[f] + q] = S[r] / a;
+                }
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const long t = (long)(i - 1) + (long)nx * ((long)(j - 1) + (long)ny * (k - 1));
+                double S[3][2][3], xf[3][2][3], xc[3] = {0., 0., 0.}, p[3];
+                for (int f = 0; f < 3; ++f) {
+                    face(f, i, j, k, S[f][0], xf[f][0]);
+                    face(f, i + (f == 0), j + (f == 1), k + (f == 2), S[f][1], xf[f][1]);
+                }
+                for (int c8 = 0; c8 < 8; ++c8) {
+                    node(i + (c8 & 1), j + ((c8 >> 1) & 1), k + ((c8 >> 2) & 1), p);
+                    for (int r = 0; r < 3; ++r) xc[r] += .125 * p[r];
+                }
+                double V = 0.;
+                for (int f = 0; f < 3; ++f) V += dot(S[f][1], xf[f][1]) - dot(S[f][0], xf[f][0]);
+                V /= 3.;
+                if (!(V > 0.)) return -MHD_ERR_PARAM;
+                ivol[t] = 1. / V;
+                for (int f = 0; f < 3; ++f) {
+                    const double a0 = norm(S[f][0]), a1 = norm(S[f][1]);
+                    for (int r = 0; r < 3; ++r) {
+                        cm[((f * 2 + 0) * 3 + r) * ncell + t] = -a0 * (xf[f][0][r] - xc[r]) / V;
+                        cm[((f * 2 + 1) * 3 + r) * ncell + t] = a1 * (xf[f][1][r] - xc[r]) / V;
+                    }
+                    double sn[3];
+                    for (int r = 0; r < 3; ++r) sn[r] = S[f][0][r] + S[f][1][r];
+                    const double ln = norm(sn);
+                    for (int r = 0; r < 3; ++r) nc[(f * 3 + r) * ncell + t] = sn[r] / ln;
+                    ih[f * ncell + t] = .5 * (a0 + a1) / V;
+                }
+            }
+    // edges: along e, owned layouts ei (nx0,ny0+1,nz0+1), ej (nx0+1,ny0,nz0+1), ek (nx0+1,ny0+1,nz0)
+    for (int e = 0; e < 3; ++e) {
+        const int n1 = nx0 + (e != 0), n2 = ny0 + (e != 1), n3 = nz0 + (e != 2);
+        const int a = (e + 1) % 3, b = (e + 2) % 3;  // a1 along axis a, a2 along axis b
+        for (int k = 1; k <= n3; ++k)
+            for (int j = 1; j <= n2; ++j)
+                for (int i = 1; i <= n1; ++i) {
+                    int I[3] = {ng + i, ng + j, ng + k};
+                    double p0[3], p1[3], dl[3], a1[3] = {0, 0, 0}, a2[3] = {0, 0, 0};
+                    node(I[0], I[1], I[2], p0);
+                    int J[3] = {I[0], I[1], I[2]};
+                    J[e] += 1;
+                    node(J[0], J[1], J[2], p1);
+                    sub(p1, p0, dl);
+                    for (int s2 = 0; s2 < 2; ++s2) {  // the edge's two end nodes
+                        int B[3] = {I[0], I[1], I[2]};
+                        B[e] += s2;
+                        int P[3] = {B[0], B[1], B[2]}, M[3] = {B[0], B[1], B[2]};
+                        P[a] += 1;
+                        M[a] -= 1;
+                        double pp[3], pm[3];
+                        node(P[0], P[1], P[2], pp);
+                        node(M[0], M[1], M[2], pm);
+                        for (int r = 0; r < 3; ++r) a1[r] += .25 * (pp[r] - pm[r]);
+                        int Q[3] = {B[0], B[1], B[2]}, N[3] = {B[0], B[1], B[2]};
+                        Q[b] += 1;
+                        N[b] -= 1;
+                        node(Q[0], Q[1], Q[2], pp);
+                        node(N[0], N[1], N[2], pm);
+                        for (int r = 0; r < 3; ++r) a2[r] += .25 * (pp[r] - pm[r]);
+                    }
+                    double S1[3], S2[3], t3[3];
+                    cross(a2, dl, S1);
+                    cross(dl, a1, S2);
+                    cross(a2, dl, t3);
+                    const double Ve = dot(a1, t3);
+                    if (!(Ve > 0.)) return -MHD_ERR_PARAM;
+                    ef[e][(long)(i - 1) + (long)n1 * ((long)(j - 1) + (long)n2 * (k - 1))] = norm(S1) * norm(S2) / Ve;
+                }
+    }
+    double* o = out;
+    auto emit = [&](const std::vector<double>& v) {
+        std::copy(v.begin(), v.end(), o);
+        o += v.size();
+    };
+    for (int f = 0; f < 3; ++f) {
+        emit(nf[f]);
+        emit(af[f]);
+        emit(iaf[f]);
+    }
+    for (int e = 0; e < 3; ++e) emit(ef[e]);
+    emit(ivol);
+    emit(cm);
+    emit(nc);
+    emit(ih);
+    return total;
+}
+
+int mhd_set_metric_nodes(mhd_ctx* c, const double* X, const double* Y, const double* Z) {
+    if (!c || !X || !Y || !Z || !c->have_geometry) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    const Dims& d = c->d;
+    const long total = mhd_metric_nodes_host(d.nx0, d.ny0, d.nz0, d.ng, X, Y, Z, nullptr);
+    std::vector<double> host((size_t)total);
+    const long r = mhd_metric_nodes_host(d.nx0, d.ny0, d.nz0, d.ng, X, Y, Z, host.data());
+    if (r < 0) return (int)-r;
+    const long nb[3] = {c->nbi, c->nbj, c->nbk};
+    const long ncell = c->ncell;
+    // device buffer: the host arrays in the same order, each 256-byte aligned
+    if (alloc_metbuf(c) != MHD_OK) return MHD_ERR_HIP;
+    auto al = [](size_t n) { return (n * sizeof(double) + 255) & ~size_t(255); };
+    size_t bytes = 0;
+    for (int f = 0; f < 3; ++f) bytes += al(3 * nb[f]) + 2 * al(nb[f]) + al(c->ne[f]);
+    bytes += al(ncell) + al(18 * ncell) + al(9 * ncell) + al(3 * ncell);
+    if (c->gmetbuf) hipFree(c->gmetbuf);
+    c->gmetbuf = nullptr;
+    if (hipMalloc(&c->gmetbuf, bytes) != hipSuccess) {
+        c->gmetbuf = nullptr;
+        return MHD_ERR_HIP;
+    }
+    char* q = c->gmetbuf;
+    const double* h = host.data();
+    bool ok = true;
+    auto put = [&](size_t n) -> const double* {
+        double* dst = (double*)q;
+        q += al(n);
+        ok = ok && hipMemcpy(dst, h, n * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+        h += n;
+        return dst;
+    };
+    GMet g{};
+    for (int f = 0; f < 3; ++f) {
+        g.nf[f] = put(3 * nb[f]);
+        g.af[f] = put(nb[f]);
+        g.iaf[f] = put(nb[f]);
+        g.nb[f] = nb[f];
+    }
+    for (int f = 0; f < 3; ++f) g.ef[f] = put(c->ne[f]);
+    g.ivol = put(ncell);
+    g.cm = put(18 * ncell);
+    g.nc = put(9 * ncell);
+    g.ih = put(3 * ncell);
+    g.ncell = ncell;
+    if (!ok) return MHD_ERR_HIP;
+    Metric m;
+    m.on = true;
+    m.mode = 2;
+    c->met = m;
+    c->gm = g;
     c->dt_fresh = false;
     if (upload_metric_spacing(c) != MHD_OK) return MHD_ERR_HIP;
     return MHD_OK;
@@ -332,6 +564,18 @@ static Src<float> sh_src(mhd_ctx* c) {
     for (int d = 0; d < 3; ++d) s.w[d] = c->w32[d];
     return s;
 }
+// the general metric's device arrays, or an empty GMet (ivol null) for the others
+static GMet gmet(const mhd_ctx* c) { return c->met.mode == 2 ? c->gm : GMet{}; }
+template <typename TH>
+static Proj<TH> proj(const mhd_ctx* c, TH* const* w) {
+    Proj<TH> pj{};
+    pj.on = c->met.on ? 1 : 0;
+    for (int q = 0; q < 9; ++q) pj.n[q] = c->met.n[q / 3][q % 3];
+    for (int d = 0; d < 3; ++d) pj.w[d] = w[d];
+    pj.nc = c->met.mode == 2 ? c->gm.nc : nullptr;
+    pj.ncell = c->ncell;
+    return pj;
+}
 static BMet bmet(mhd_ctx* c, bool w64, bool w32) {
     BMet m{};
     m.on = c->met.on ? 1 : 0;
@@ -343,6 +587,7 @@ static BMet bmet(mhd_ctx* c, bool w64, bool w32) {
         m.w[d] = w64 ? c->w64[d] : nullptr;
         m.ws[d] = w32 ? c->w32[d] : nullptr;
     }
+    m.g = gmet(c);
     return m;
 }
 static bool mixed(const mhd_ctx* c) { return c->p.precision == MHD_MIXED; }
@@ -404,7 +649,7 @@ static void do_bxyz(mhd_ctx* c, bool want_vf, bool want_shadow, bool want_dt, bo
 static void do_dt(mhd_ctx* c, bool set_tau) {
     if (!c->dt_fresh) {
         launch(c, MHD_K_DT, blocks(c->ncell), dim3(kBlock), k_dt_partial, c->d, c->p.gam, c->p.cfl, cur_state(c),
-               sdx(c), sdy(c), sdz(c), c->dtpart, c->ncell);
+               sdx(c), sdy(c), sdz(c), gmet(c), c->dtpart, c->ncell);
         launch(c, MHD_K_DT, dim3(1), dim3(1024), k_dt_final, (const double*)c->dtpart, blocks(c->ncell).x * 1L,
                c->scal + SC_DTCFL, set_tau ? c->scal + SC_TAU : (double*)nullptr);
         return;
@@ -428,7 +673,7 @@ static void do_update_t(mhd_ctx* c, double* const* out) {
     PrevRK pv{c->rhop, c->mxp, c->myp, c->mzp, c->engp};
     FluxSet<TF> F{(const TF*)c->flux[0], (const TF*)c->flux[1], (const TF*)c->flux[2], c->nf[0], c->nf[1], c->nf[2]};
     launch(c, MHD_K_UPDATE, blocks(c->ninterior), dim3(kBlock), k_update<NST, ST, TF>, c->d, c->p.gam - 1.,
-           (const double*)(c->scal + SC_TAU), s, so, pv, F, sdx(c), sdy(c), sdz(c), c->ninterior);
+           (const double*)(c->scal + SC_TAU), s, so, pv, F, sdx(c), sdy(c), sdz(c), gmet(c), c->ninterior);
     CFaces b{c->bi, c->bj, c->bk};
     Faces bo{out[8], out[9], out[10]};
     CFaces bp{c->bip, c->bjp, c->bkp};
@@ -437,7 +682,7 @@ static void do_update_t(mhd_ctx* c, double* const* out) {
     const CtFactor ctf{{c->met.ct[0], c->met.ct[1], c->met.ct[2]}};
     launch(c, MHD_K_CT, blocks(nct), dim3(kBlock), k_ct<NST, ST, TF>, c->d, (const double*)(c->scal + SC_TAU), ctf, b,
            bo, bp, (const TF*)c->emf[0], (const TF*)c->emf[1], (const TF*)c->emf[2], (const double*)c->dx,
-           (const double*)c->dy, (const double*)c->dz);
+           (const double*)c->dy, (const double*)c->dz, gmet(c));
 }
 
 template <int NST, int ST>
@@ -594,10 +839,7 @@ static void step_ab(mhd_ctx* c) {
         // extrapolated state straight into the fp32 stencil copies (sh order: .. bz vf bi bj bk)
         for (int a = 0; a < 8; ++a) s.xh[a] = c->sh[a];
         for (int a = 8; a < 11; ++a) s.xh[a] = c->sh[a + 1];
-        Proj<float> pj{};
-        pj.on = c->met.on ? 1 : 0;
-        for (int q = 0; q < 9; ++q) pj.n[q] = c->met.n[q / 3][q % 3];
-        for (int d = 0; d < 3; ++d) pj.w[d] = c->w32[d];
+        const Proj<float> pj = proj<float>(c, c->w32);
         const FaceB fb{c->bi, c->bj, c->bk, c->xp[8], c->xp[9], c->xp[10]};
         launch(c, MHD_K_PREV, blocks(c->nmax), dim3(kBlock), k_predict_ab_mixed, c->d, s, fb, bmet(c, false, false),
                (const double*)(c->scal + SC_TAU), c->sh[8], pj, nc, nbi, nbj, nbk);
@@ -610,10 +852,7 @@ static void step_ab(mhd_ctx* c) {
             s.xp[a] = c->xp[a];
             s.xh[a] = c->xh[a];
         }
-        Proj<double> pj{};
-        pj.on = c->met.on ? 1 : 0;
-        for (int q = 0; q < 9; ++q) pj.n[q] = c->met.n[q / 3][q % 3];
-        for (int d = 0; d < 3; ++d) pj.w[d] = c->wh64[d];
+        const Proj<double> pj = proj<double>(c, c->wh64);
         launch(c, MHD_K_PREV, blocks(c->nmax), dim3(kBlock), k_predict_ab<double>, s,
                (const double*)(c->scal + SC_TAU), c->vfh, pj, nc, nbi, nbj, nbk);
         launch(c, MHD_K_PREV, dim3(1), dim3(64), k_tau0_from_tau, c->scal + SC_TAU);

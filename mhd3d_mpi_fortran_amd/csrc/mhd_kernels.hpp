@@ -136,6 +136,22 @@ __device__ __forceinline__ void face_flux(const Coef<R>& c, const R* l, const R*
 // through it is the Gaussian-split flux in vector form along n_d.  With n_d
 // a coordinate axis this is the reference's get_flux_stress term for term.
 // ------------------------------------------------------------------------
+// general metric (mhd_set_metric_nodes): per-face and per-cell geometry
+// from the node coordinates, binary64, device arrays.  Face arrays use the
+// ghost-inclusive face layouts of bi, bj, bk; edge arrays the owned-edge
+// layouts of ei, ej, ek; cell arrays the ghost-inclusive cell layout.
+struct GMet {
+    const double* nf[3];   // unit normals of face family d, 3 components (SoA, stride = face count)
+    const double* af[3];   // face areas |S|
+    const double* iaf[3];  // 1/|S|
+    const double* ef[3];   // EMF factor |S1||S2|/V of the local edge basis, per owned edge (E . dl)
+    const double* ivol;    // 1/V
+    const double* cm;      // cell B = sum_d beta_d(-) cm[d][0] + beta_d(+) cm[d][1]; 18 per cell (SoA)
+    const double* nc;      // cell normals of the 3 face families, 9 per cell (SoA)
+    const double* ih;      // CFL: 0.5 (|S-| + |S+|) / V per family, 3 per cell (SoA)
+    long ncell, nb[3];
+};
+
 template <typename R>
 struct Normals {
     R n[3][3];  // n[d] = unit normal of the faces normal to computational axis d
@@ -190,10 +206,25 @@ __device__ __forceinline__ void face_flux_met(const Coef<R>& c, const R* l, cons
     for (int q = 0; q < 8; ++q) F[q * nface + t] = TF(fx[q]);
 }
 
-// MET = false: the reference's axis-aligned flux; true: along the metric's normals
-template <typename R, int M, int D, typename TI, typename TF, bool MET>
-__global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<TI> q, Normals<R> nm, TF* __restrict__ F,
-                                                 long nface) {
+// normal of face (i, j, k) (owned, 1-based) of family D from a general
+// metric's per-face array (ghost-inclusive face layout)
+template <typename R, int D>
+__device__ __forceinline__ void gen_normal(const Dims& d, const GMet& g, int i, int j, int k, R* n) {
+    const ptrdiff_t f = D == 1   ? ix(d.nx + 1, d.ny, d.ng + i, d.ng + j, d.ng + k)
+                        : D == 2 ? ix(d.nx, d.ny + 1, d.ng + i, d.ng + j, d.ng + k)
+                                 : ix(d.nx, d.ny, d.ng + i, d.ng + j, d.ng + k);
+    const double* a = g.nf[D - 1];
+    const long nb = g.nb[D - 1];
+    n[0] = R(a[f]);
+    n[1] = R(a[nb + f]);
+    n[2] = R(a[2 * nb + f]);
+}
+
+// MET = 0: the reference's axis-aligned flux; 1: along the affine metric's
+// normals; 2: along the general metric's per-face normals
+template <typename R, int M, int D, typename TI, typename TF, int MET>
+__global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<TI> q, Normals<R> nm, GMet gm,
+                                                 TF* __restrict__ F, long nface) {
     const long t = gtid_xcd();
     if (t >= nface) return;
     const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
@@ -210,8 +241,15 @@ __global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<TI> q, 
     recon_line<R, M>(q.bx, base, st, c, l[5], r[5]);
     recon_line<R, M>(q.by, base, st, c, l[6], r[6]);
     recon_line<R, M>(q.bz, base, st, c, l[7], r[7]);
-    if (MET) face_flux_met<R, TF>(c, l, r, nm.n[D - 1], F, t, nface);
-    else face_flux<R, D, TF>(c, l, r, F, t, nface);
+    if (MET == 2) {
+        R n[3];
+        gen_normal<R, D>(d, gm, i, j, k, n);
+        face_flux_met<R, TF>(c, l, r, n, F, t, nface);
+    } else if (MET == 1) {
+        face_flux_met<R, TF>(c, l, r, nm.n[D - 1], F, t, nface);
+    } else {
+        face_flux<R, D, TF>(c, l, r, F, t, nface);
+    }
 }
 
 // ------------------------------------------------------------------------

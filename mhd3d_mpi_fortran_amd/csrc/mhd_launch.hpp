@@ -18,9 +18,11 @@ struct Src {
 };
 
 // affine metric x_phys = A x_comp (mhd_set_metric); on = false is the
-// reference's tensor-product grid, for which every factor below is exactly 1
+// reference's tensor-product grid, for which every factor below is exactly 1.
+// mode 2: the general metric of GMet (mhd_set_metric_nodes)
 struct Metric {
     bool on = false;
+    int mode = 0;  // 0 tensor product, 1 affine, 2 general
     double A[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};  // row-major
     double n[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};  // unit face normals per computational axis
     double g[3] = {1, 1, 1};    // |C e_d| / det A: flux divergence factor (update, CFL)
@@ -87,6 +89,8 @@ struct mhd_ctx {
     std::vector<double> hdx, hdy, hdz;
     double *dxm = nullptr, *dym = nullptr, *dzm = nullptr;
     char* metbuf = nullptr;
+    char* gmetbuf = nullptr;
+    mhd::GMet gm{};
     double* w64[3] = {};
     double* wh64[3] = {};
     float* w32[3] = {};

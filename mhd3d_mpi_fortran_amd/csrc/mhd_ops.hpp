@@ -8,23 +8,24 @@
 
 namespace mhd {
 
-template <typename R, typename TI, typename TF, int M, bool MET>
+template <typename R, typename TI, typename TF, int M, int MET>
 void flux_launch_mm(mhd_ctx* c, Src<TI> s) {
     const Coef<R> co = make_coef<R>(c);
     const Prim<TI> q{s.rho, s.vx, s.vy, s.vz, s.p, s.bx, s.by, s.bz};
     const Normals<R> nm = make_normals<R>(c);
-    launch(c, MHD_K_FLUX, blocks(c->nf[0]), dim3(kBlock), k_flux<R, M, 1, TI, TF, MET>, c->d, co, q, nm,
+    launch(c, MHD_K_FLUX, blocks(c->nf[0]), dim3(kBlock), k_flux<R, M, 1, TI, TF, MET>, c->d, co, q, nm, c->gm,
            (TF*)c->flux[0], c->nf[0]);
-    launch(c, MHD_K_FLUX, blocks(c->nf[1]), dim3(kBlock), k_flux<R, M, 2, TI, TF, MET>, c->d, co, q, nm,
+    launch(c, MHD_K_FLUX, blocks(c->nf[1]), dim3(kBlock), k_flux<R, M, 2, TI, TF, MET>, c->d, co, q, nm, c->gm,
            (TF*)c->flux[1], c->nf[1]);
-    launch(c, MHD_K_FLUX, blocks(c->nf[2]), dim3(kBlock), k_flux<R, M, 3, TI, TF, MET>, c->d, co, q, nm,
+    launch(c, MHD_K_FLUX, blocks(c->nf[2]), dim3(kBlock), k_flux<R, M, 3, TI, TF, MET>, c->d, co, q, nm, c->gm,
            (TF*)c->flux[2], c->nf[2]);
 }
 
 template <typename R, typename TI, typename TF, int M>
 void flux_launch_m(mhd_ctx* c, Src<TI> s) {
-    if (c->met.on) flux_launch_mm<R, TI, TF, M, true>(c, s);
-    else flux_launch_mm<R, TI, TF, M, false>(c, s);
+    if (c->met.mode == 2) flux_launch_mm<R, TI, TF, M, 2>(c, s);
+    else if (c->met.on) flux_launch_mm<R, TI, TF, M, 1>(c, s);
+    else flux_launch_mm<R, TI, TF, M, 0>(c, s);
 }
 
 template <typename R, typename TI, typename TF>
@@ -52,7 +53,7 @@ void emf_launch_k(mhd_ctx* c, EmfIn<TI> in) {
     const long nm = K == 3 ? d.ny0 + 1 : d.nz0 + 1;
     const long n = ii * no * ((nm + T - 1) / T);
     launch(c, MHD_K_EMF, blocks(n), dim3(kBlock), k_emf_march<R, M, K, T, TI, TF>, d, make_coef<R>(c), in,
-           R(c->met.ef[K - 1]), (TF*)c->emf[K - 1], n);
+           R(c->met.ef[K - 1]), c->met.mode == 2 ? c->gm.ef[K - 1] : (const double*)nullptr, (TF*)c->emf[K - 1], n);
 }
 
 // edge k: (v1, v2, b1, b2) = (vy, vz, bj, bk), (vz, vx, bk, bi), (vx, vy, bi, bj);

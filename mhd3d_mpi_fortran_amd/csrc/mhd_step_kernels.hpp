@@ -30,11 +30,12 @@ struct FluxSet {
     long nfx, nfy, nfz;
 };
 
+// gm.ivol != null: general metric, div = (sum_f F_f |S_f| out) / V
 template <int NST, int ST, typename TF>
 __global__ void __launch_bounds__(kBlock) k_update(Dims d, double gm1, const double* __restrict__ tau_p, CCells s,
                                                   Cells so, PrevRK pv, FluxSet<TF> F, const double* __restrict__ dx,
                                                   const double* __restrict__ dy, const double* __restrict__ dz,
-                                                  long ncell) {
+                                                  GMet gm, long ncell) {
     const long t = gtid_xcd();
     if (t >= ncell) return;
     int i, j, k;
@@ -46,7 +47,22 @@ __global__ void __launch_bounds__(kBlock) k_update(Dims d, double gm1, const dou
     const ptrdiff_t fx0 = ix(d.nx0 + 1, d.ny0, i, j, k);
     const ptrdiff_t fy0 = ix(d.nx0, d.ny0 + 1, i, j, k), fy1 = fy0 + d.nx0;
     const ptrdiff_t fz0 = ix(d.nx0, d.ny0, i, j, k), fz1 = fz0 + (ptrdiff_t)d.nx0 * d.ny0;
+    double ar[6] = {0., 0., 0., 0., 0., 0.}, iv = 0.;
+    if (gm.ivol) {
+        const ptrdiff_t gi = ix(d.nx + 1, d.ny, ng + i, ng + j, ng + k), gj = ix(d.nx, d.ny + 1, ng + i, ng + j, ng + k);
+        ar[0] = gm.af[0][gi];
+        ar[1] = gm.af[0][gi + 1];
+        ar[2] = gm.af[1][gj];
+        ar[3] = gm.af[1][gj + d.nx];
+        ar[4] = gm.af[2][q];
+        ar[5] = gm.af[2][q + (ptrdiff_t)d.nx * d.ny];
+        iv = gm.ivol[q];
+    }
     auto div = [&](int f) -> double {
+        if (gm.ivol)
+            return ((double(F.x[f * F.nfx + fx0 + 1]) * ar[1] - double(F.x[f * F.nfx + fx0]) * ar[0]) +
+                    (double(F.y[f * F.nfy + fy1]) * ar[3] - double(F.y[f * F.nfy + fy0]) * ar[2]) +
+                    (double(F.z[f * F.nfz + fz1]) * ar[5] - double(F.z[f * F.nfz + fz0]) * ar[4])) * iv;
         return (double(F.x[f * F.nfx + fx0 + 1]) - double(F.x[f * F.nfx + fx0])) / dxi +
                (double(F.y[f * F.nfy + fy1]) - double(F.y[f * F.nfy + fy0])) / dyj +
                (double(F.z[f * F.nfz + fz1]) - double(F.z[f * F.nfz + fz0])) / dzk;
@@ -122,12 +138,14 @@ struct CtFactor {
     double f[3];
 };
 
+// gm.ivol != null: general metric, the EMFs are E . dl and the face update
+// divides by the face's area
 template <int NST, int ST, typename TF>
 __global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict__ tau_p, CtFactor ctf, CFaces b,
                                               Faces bo, CFaces bp, const TF* __restrict__ ei,
                                               const TF* __restrict__ ej, const TF* __restrict__ ek,
                                               const double* __restrict__ dx, const double* __restrict__ dy,
-                                              const double* __restrict__ dz) {
+                                              const double* __restrict__ dz, GMet gm) {
     const long t = gtid();
     const int nx0 = d.nx0, ny0 = d.ny0, nz0 = d.nz0, ng = d.ng;
     const long nbi = (long)(nx0 + 1) * ny0 * nz0, nbj = (long)nx0 * (ny0 + 1) * nz0,
@@ -147,7 +165,8 @@ __global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict_
         const double dej = double(ej[e + (ptrdiff_t)(nx0 + 1) * ny0]) - double(ej[e]);
         q = ix(d.nx + 1, d.ny, ng + i, ng + j, ng + k);
         old = b.bi[q];
-        val = old - tau * ctf.f[0] * (dek / dy[ng + j - 1] - dej / dz[ng + k - 1]);
+        val = gm.ivol ? old - tau * gm.iaf[0][q] * (dek - dej)
+                      : old - tau * ctf.f[0] * (dek / dy[ng + j - 1] - dej / dz[ng + k - 1]);
         out = bo.bi;
         prev = bp.bi;
     } else if (t < nbi + nbj) {
@@ -158,7 +177,8 @@ __global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict_
         const double dek = double(ek[e + 1]) - double(ek[e]);
         q = ix(d.nx, d.ny + 1, ng + i, ng + j, ng + k);
         old = b.bj[q];
-        val = old - tau * ctf.f[1] * (dei / dz[ng + k - 1] - dek / dx[ng + i - 1]);
+        val = gm.ivol ? old - tau * gm.iaf[1][q] * (dei - dek)
+                      : old - tau * ctf.f[1] * (dei / dz[ng + k - 1] - dek / dx[ng + i - 1]);
         out = bo.bj;
         prev = bp.bj;
     } else {
@@ -169,7 +189,8 @@ __global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict_
         const double dei = double(ei[e + nx0]) - double(ei[e]);
         q = ix(d.nx, d.ny, ng + i, ng + j, ng + k);
         old = b.bk[q];
-        val = old - tau * ctf.f[2] * (dej / dx[ng + i - 1] - dei / dy[ng + j - 1]);
+        val = gm.ivol ? old - tau * gm.iaf[2][q] * (dej - dei)
+                      : old - tau * ctf.f[2] * (dej / dx[ng + i - 1] - dei / dy[ng + j - 1]);
         out = bo.bk;
         prev = bp.bk;
     }
@@ -292,7 +313,30 @@ struct BMet {
     double n[9];
     double* w[3];
     float* ws[3];
+    GMet g;   // g.ivol != null: general metric (cell B from the 6 faces, per-cell normals and CFL spacing)
 };
+
+// cell B of cell t from its 6 faces through a general metric's coefficients
+__device__ __forceinline__ void gen_cellb(const GMet& g, long t, const double* beta, double* B) {
+    const long n = g.ncell;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        double acc = 0.;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) acc += beta[f] * g.cm[(f * 3 + r) * n + t];
+        B[r] = acc;
+    }
+}
+
+__device__ __forceinline__ double cfl_dt_gen(double gam, double cfl, double u, double v, double w, double x, double y,
+                                             double z, double r, double pp, const GMet& g, long t) {
+    const double vv = sqrt(u * u + v * v + w * w);
+    const double btot = sqrt(x * x + y * y + z * z);
+    const double valf = btot / sqrt(r);
+    const double vs = sqrt(gam * pp / r);
+    const double vcfl = vv + sqrt(valf * valf + vs * vs);
+    return cfl / (vcfl * g.ih[t] + vcfl * g.ih[g.ncell + t] + vcfl * g.ih[2 * g.ncell + t]);
+}
 
 __device__ __forceinline__ double cfl_dt(double gam, double cfl, double u, double v, double w, double x, double y,
                                          double z, double r, double pp, double dxi, double dyj, double dzk) {
@@ -315,11 +359,29 @@ __global__ void __launch_bounds__(kBlock) k_bxyz(Dims d, double gam, double cfl,
     if (t < ncell) {
         int i, j, k;
         unflat(t, d.nx, d.ny, i, j, k);
-        double x = .5 * (b.bi[ix(d.nx + 1, d.ny, i + 1, j, k)] + b.bi[ix(d.nx + 1, d.ny, i, j, k)]);
-        double y = .5 * (b.bj[ix(d.nx, d.ny + 1, i, j + 1, k)] + b.bj[ix(d.nx, d.ny + 1, i, j, k)]);
-        double z = .5 * (b.bk[t + (ptrdiff_t)d.nx * d.ny] + b.bk[t]);
+        const double bi0 = b.bi[ix(d.nx + 1, d.ny, i, j, k)], bi1 = b.bi[ix(d.nx + 1, d.ny, i + 1, j, k)];
+        const double bj0 = b.bj[ix(d.nx, d.ny + 1, i, j, k)], bj1 = b.bj[ix(d.nx, d.ny + 1, i, j + 1, k)];
+        const double bk0 = b.bk[t], bk1 = b.bk[t + (ptrdiff_t)d.nx * d.ny];
+        double x = .5 * (bi1 + bi0);
+        double y = .5 * (bj1 + bj0);
+        double z = .5 * (bk1 + bk0);
         const double u = s.vx[t], v = s.vy[t], w = s.vz[t], r = s.rho[t], pp = s.p[t];
-        if (met.on) {
+        if (met.g.ivol) {
+            const double beta[6] = {bi0, bi1, bj0, bj1, bk0, bk1};
+            double B[3];
+            gen_cellb(met.g, t, beta, B);
+            x = B[0];
+            y = B[1];
+            z = B[2];
+            const long n = met.g.ncell;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const double wn = met.g.nc[(3 * a) * n + t] * u + met.g.nc[(3 * a + 1) * n + t] * v +
+                                  met.g.nc[(3 * a + 2) * n + t] * w;
+                if (met.w[a]) met.w[a][t] = wn;
+                if (met.ws[a]) met.ws[a][t] = float(wn);
+            }
+        } else if (met.on) {
             const double fx = x, fy = y, fz = z;
             x = met.M[0] * fx + met.M[1] * fy + met.M[2] * fz;
             y = met.M[3] * fx + met.M[4] * fy + met.M[5] * fz;
@@ -351,7 +413,9 @@ __global__ void __launch_bounds__(kBlock) k_bxyz(Dims d, double gam, double cfl,
                 sh.vf[t] = float(f);
             }
         }
-        if (part) dt = cfl_dt(gam, cfl, u, v, w, x, y, z, r, pp, dx[i - 1], dy[j - 1], dz[k - 1]);
+        if (part)
+            dt = met.g.ivol ? cfl_dt_gen(gam, cfl, u, v, w, x, y, z, r, pp, met.g, t)
+                            : cfl_dt(gam, cfl, u, v, w, x, y, z, r, pp, dx[i - 1], dy[j - 1], dz[k - 1]);
     }
     if (sh.bi) {
         if (t < nbi) sh.bi[t] = float(b.bi[t]);
@@ -379,15 +443,17 @@ __global__ void __launch_bounds__(kBlock) k_vf(Prim<double> q, double* __restric
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_dt_partial(Dims d, double gam, double cfl, Prim<double> q,
                                                       const double* __restrict__ dx, const double* __restrict__ dy,
-                                                      const double* __restrict__ dz, double* __restrict__ part,
-                                                      long ncell) {
+                                                      const double* __restrict__ dz, GMet gm,
+                                                      double* __restrict__ part, long ncell) {
     const long t = gtid();
     double dt = __builtin_huge_val();
     if (t < ncell) {
         int i, j, k;
         unflat(t, d.nx, d.ny, i, j, k);
-        dt = cfl_dt(gam, cfl, q.vx[t], q.vy[t], q.vz[t], q.bx[t], q.by[t], q.bz[t], q.rho[t], q.p[t], dx[i - 1],
-                    dy[j - 1], dz[k - 1]);
+        dt = gm.ivol ? cfl_dt_gen(gam, cfl, q.vx[t], q.vy[t], q.vz[t], q.bx[t], q.by[t], q.bz[t], q.rho[t], q.p[t],
+                                  gm, t)
+                     : cfl_dt(gam, cfl, q.vx[t], q.vy[t], q.vz[t], q.bx[t], q.by[t], q.bz[t], q.rho[t], q.p[t],
+                              dx[i - 1], dy[j - 1], dz[k - 1]);
     }
     dt = block_min(dt);
     if (threadIdx.x == 0) part[blockIdx.x] = dt;
@@ -467,7 +533,20 @@ struct Proj {
     int on;
     double n[9];
     TH* w[3];
+    const double* nc;  // general metric: per-cell normals (9 per cell, SoA), else null
+    long ncell;
 };
+
+template <typename TH>
+__device__ __forceinline__ void project(const Proj<TH>& pj, long t, double u, double v, double w) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double n0 = pj.nc ? pj.nc[(3 * a) * pj.ncell + t] : pj.n[3 * a];
+        const double n1 = pj.nc ? pj.nc[(3 * a + 1) * pj.ncell + t] : pj.n[3 * a + 1];
+        const double n2 = pj.nc ? pj.nc[(3 * a + 2) * pj.ncell + t] : pj.n[3 * a + 2];
+        pj.w[a][t] = TH(n0 * u + n1 * v + n2 * w);
+    }
+}
 
 template <typename TH>
 __global__ void __launch_bounds__(kBlock) k_predict_ab(AB11<TH> s, const double* __restrict__ tau_p,
@@ -484,10 +563,7 @@ __global__ void __launch_bounds__(kBlock) k_predict_ab(AB11<TH> s, const double*
             s.xh[a][t] = TH(h[a]);
         }
         vfh[t] = TH(sqrt(h[1] * h[1] + h[2] * h[2] + h[3] * h[3]) + sqrt((h[5] * h[5] + h[6] * h[6] + h[7] * h[7]) / h[0]));
-        if (pj.on) {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) pj.w[a][t] = TH(pj.n[3 * a] * h[1] + pj.n[3 * a + 1] * h[2] + pj.n[3 * a + 2] * h[3]);
-        }
+        if (pj.on) project(pj, t, h[1], h[2], h[3]);
     }
     const long nf[3] = {nbi, nbj, nbk};
 #pragma unroll
@@ -530,7 +606,12 @@ __global__ void __launch_bounds__(kBlock) k_predict_ab_mixed(Dims d, AB11<float>
         const double ax = .5 * (ext(fb.bi, fb.bip, fi + 1) + ext(fb.bi, fb.bip, fi));
         const double ay = .5 * (ext(fb.bj, fb.bjp, fj + d.nx) + ext(fb.bj, fb.bjp, fj));
         const double az = .5 * (ext(fb.bk, fb.bkp, t + (ptrdiff_t)d.nx * d.ny) + ext(fb.bk, fb.bkp, t));
-        if (met.on) {
+        if (met.g.ivol) {
+            const double beta[6] = {ext(fb.bi, fb.bip, fi), ext(fb.bi, fb.bip, fi + 1), ext(fb.bj, fb.bjp, fj),
+                                    ext(fb.bj, fb.bjp, fj + d.nx), ext(fb.bk, fb.bkp, t),
+                                    ext(fb.bk, fb.bkp, t + (ptrdiff_t)d.nx * d.ny)};
+            gen_cellb(met.g, t, beta, h + 5);
+        } else if (met.on) {
             h[5] = met.M[0] * ax + met.M[1] * ay + met.M[2] * az;
             h[6] = met.M[3] * ax + met.M[4] * ay + met.M[5] * az;
             h[7] = met.M[6] * ax + met.M[7] * ay + met.M[8] * az;
@@ -542,11 +623,7 @@ __global__ void __launch_bounds__(kBlock) k_predict_ab_mixed(Dims d, AB11<float>
 #pragma unroll
         for (int a = 5; a < 8; ++a) s.xh[a][t] = float(h[a]);
         vfh[t] = float(sqrt(h[1] * h[1] + h[2] * h[2] + h[3] * h[3]) + sqrt((h[5] * h[5] + h[6] * h[6] + h[7] * h[7]) / h[0]));
-        if (pj.on) {
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-                pj.w[a][t] = float(pj.n[3 * a] * h[1] + pj.n[3 * a + 1] * h[2] + pj.n[3 * a + 2] * h[3]);
-        }
+        if (pj.on) project(pj, t, h[1], h[2], h[3]);
     }
     const long nf[3] = {nbi, nbj, nbk};
 #pragma unroll

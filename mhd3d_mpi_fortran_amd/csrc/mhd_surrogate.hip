@@ -470,8 +470,8 @@ __global__ void __launch_bounds__(kBlock) k_infer(const float* __restrict__ x, l
 // ---------------------------------------------------------------------------
 constexpr int SUR_GROUPS = 4;
 
-template <int D, bool MET>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_flux_sur(Dims d, Coef<float> c, Prim<float> q, Normals<float> nm, float theta,
+template <int D, int MET>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_flux_sur(Dims d, Coef<float> c, Prim<float> q, Normals<float> nm, GMet gm, float theta,
                                                     const unsigned short* __restrict__ wbf,
                                                     const float* __restrict__ bias, float* __restrict__ F, long nface,
                                                     int* __restrict__ fix, unsigned long long* __restrict__ nfix,
@@ -570,15 +570,22 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
             }
         }
         if (live && !fb) {
-            if (MET) face_flux_met<float, float>(c, lv, rv, nm.n[D - 1], F, t, nface);
-            else face_flux<float, D>(c, lv, rv, F, t, nface);
+            if (MET == 2) {
+                float n[3];
+                gen_normal<float, D>(d, gm, i, j, k, n);
+                face_flux_met<float, float>(c, lv, rv, n, F, t, nface);
+            } else if (MET == 1) {
+                face_flux_met<float, float>(c, lv, rv, nm.n[D - 1], F, t, nface);
+            } else {
+                face_flux<float, D>(c, lv, rv, F, t, nface);
+            }
         }
     }
 }
 
 // PDM flux of the listed fallback faces (fp32, as the plain mixed path)
-template <int D, bool MET>
-__global__ void __launch_bounds__(kBlock) k_flux_fix(Dims d, Coef<float> c, Prim<float> q, Normals<float> nm,
+template <int D, int MET>
+__global__ void __launch_bounds__(kBlock) k_flux_fix(Dims d, Coef<float> c, Prim<float> q, Normals<float> nm, GMet gm,
                                                     const int* __restrict__ fix,
                                                     const unsigned long long* __restrict__ nfix,
                                                     float* __restrict__ F, long nface) {
@@ -599,8 +606,15 @@ __global__ void __launch_bounds__(kBlock) k_flux_fix(Dims d, Coef<float> c, Prim
         recon_line<float, PDM>(q.bx, base, st, c, l[5], r[5]);
         recon_line<float, PDM>(q.by, base, st, c, l[6], r[6]);
         recon_line<float, PDM>(q.bz, base, st, c, l[7], r[7]);
-        if (MET) face_flux_met<float, float>(c, l, r, nm.n[D - 1], F, t, nface);
-        else face_flux<float, D>(c, l, r, F, t, nface);
+        if (MET == 2) {
+            float n[3];
+            gen_normal<float, D>(d, gm, i, j, k, n);
+            face_flux_met<float, float>(c, l, r, n, F, t, nface);
+        } else if (MET == 1) {
+            face_flux_met<float, float>(c, l, r, nm.n[D - 1], F, t, nface);
+        } else {
+            face_flux<float, D>(c, l, r, F, t, nface);
+        }
     }
 }
 
@@ -652,7 +666,7 @@ void surrogate_free(mhd_ctx* c) {
     c->sur = nullptr;
 }
 
-template <bool MET>
+template <int MET>
 static void flux_sur_launch(mhd_ctx* c, SurrogateState* S, Src<float> s) {
     const Coef<float> co = make_coef<float>(c);
     const Normals<float> nm = make_normals<float>(c);
@@ -666,19 +680,19 @@ static void flux_sur_launch(mhd_ctx* c, SurrogateState* S, Src<float> s) {
                               (unsigned)((c->nf[2] + sur::SUR_GROUPS * kBlock - 1) / (sur::SUR_GROUPS * kBlock))};
     const unsigned short* wbf = S->wbf;
     const float* bias = S->bias;
-    launch(c, MHD_K_FLUX, dim3(gsur[0]), dim3(kBlock), sur::k_flux_sur<1, MET>, c->d, co, q, nm, theta, wbf, bias,
+    launch(c, MHD_K_FLUX, dim3(gsur[0]), dim3(kBlock), sur::k_flux_sur<1, MET>, c->d, co, q, nm, c->gm, theta, wbf, bias,
            (float*)c->flux[0], c->nf[0], fx[0], S->nfix + 0, st);
-    launch(c, MHD_K_FLUX, dim3(gsur[1]), dim3(kBlock), sur::k_flux_sur<2, MET>, c->d, co, q, nm, theta, wbf, bias,
+    launch(c, MHD_K_FLUX, dim3(gsur[1]), dim3(kBlock), sur::k_flux_sur<2, MET>, c->d, co, q, nm, c->gm, theta, wbf, bias,
            (float*)c->flux[1], c->nf[1], fx[1], S->nfix + 1, st);
-    launch(c, MHD_K_FLUX, dim3(gsur[2]), dim3(kBlock), sur::k_flux_sur<3, MET>, c->d, co, q, nm, theta, wbf, bias,
+    launch(c, MHD_K_FLUX, dim3(gsur[2]), dim3(kBlock), sur::k_flux_sur<3, MET>, c->d, co, q, nm, c->gm, theta, wbf, bias,
            (float*)c->flux[2], c->nf[2], fx[2], S->nfix + 2, st);
     // the fallback lists: a fixed grid walks whatever length each list has
     const dim3 gfix(1024);
-    launch(c, MHD_K_SURR, gfix, dim3(kBlock), sur::k_flux_fix<1, MET>, c->d, co, q, nm, (const int*)fx[0],
+    launch(c, MHD_K_SURR, gfix, dim3(kBlock), sur::k_flux_fix<1, MET>, c->d, co, q, nm, c->gm, (const int*)fx[0],
            (const unsigned long long*)(S->nfix + 0), (float*)c->flux[0], c->nf[0]);
-    launch(c, MHD_K_SURR, gfix, dim3(kBlock), sur::k_flux_fix<2, MET>, c->d, co, q, nm, (const int*)fx[1],
+    launch(c, MHD_K_SURR, gfix, dim3(kBlock), sur::k_flux_fix<2, MET>, c->d, co, q, nm, c->gm, (const int*)fx[1],
            (const unsigned long long*)(S->nfix + 1), (float*)c->flux[1], c->nf[1]);
-    launch(c, MHD_K_SURR, gfix, dim3(kBlock), sur::k_flux_fix<3, MET>, c->d, co, q, nm, (const int*)fx[2],
+    launch(c, MHD_K_SURR, gfix, dim3(kBlock), sur::k_flux_fix<3, MET>, c->d, co, q, nm, c->gm, (const int*)fx[2],
            (const unsigned long long*)(S->nfix + 2), (float*)c->flux[2], c->nf[2]);
 }
 
@@ -699,8 +713,9 @@ void launch_flux_surrogate(mhd_ctx* c, Src<float> s) {
             return;
         }
     }
-    if (c->met.on) flux_sur_launch<true>(c, S, s);
-    else flux_sur_launch<false>(c, S, s);
+    if (c->met.mode == 2) flux_sur_launch<2>(c, S, s);
+    else if (c->met.on) flux_sur_launch<1>(c, S, s);
+    else flux_sur_launch<0>(c, S, s);
 }
 
 static int sync_or(mhd_ctx* c) {

@@ -48,6 +48,8 @@ SIGNATURES = {
     "mhd_error_string": (C.c_char_p, [C.c_int]),
     "mhd_set_geometry": (C.c_int, [C.c_void_p, _D, _D, _D]),
     "mhd_set_metric": (C.c_int, [C.c_void_p, _D]),
+    "mhd_set_metric_nodes": (C.c_int, [C.c_void_p, _D, _D, _D]),
+    "mhd_metric_nodes_host": (C.c_long, [C.c_int, C.c_int, C.c_int, C.c_int, _D, _D, _D, _D]),
     "mhd_upload_state": (C.c_int, [C.c_void_p] + [_D] * 8),
     "mhd_download_state": (C.c_int, [C.c_void_p] + [_D] * 11),
     "mhd_initialize": (C.c_int, [C.c_void_p, _D]),
@@ -142,13 +144,28 @@ class Solver:
         dy = np.ascontiguousarray(grid.dy)
         dz = np.ascontiguousarray(grid.dz)
         _ck(lib().mhd_set_geometry(self.h, _p(dx), _p(dy), _p(dz)), "mhd_set_geometry")
-        if getattr(grid, "A", None) is not None:
+        if getattr(grid, "nodes", None) is not None:
+            self.set_metric_nodes(*grid.nodes)
+        elif getattr(grid, "A", None) is not None:
             self.set_metric(grid.A)
 
     def set_metric(self, A):
         """affine metric x_phys = A x_comp (row-major 3x3)"""
         a = np.ascontiguousarray(np.asarray(A, dtype=np.float64).reshape(3, 3))
         _ck(lib().mhd_set_metric(self.h, _p(a)), "mhd_set_metric")
+
+    def set_metric_nodes(self, X, Y, Z):
+        """general metric from the node coordinates, each (nx+1, ny+1, nz+1)
+        ghost-inclusive, node (i,j,k) the lower corner of cell (i,j,k)"""
+        g = self.grid
+        shp = (g.nx + 1, g.ny + 1, g.nz + 1)
+        xs = []
+        for a in (X, Y, Z):
+            a = np.asarray(a, dtype=np.float64)
+            if a.shape != shp:
+                raise ValueError("node array shape %s, expected %s" % (a.shape, shp))
+            xs.append(np.asfortranarray(a))
+        _ck(lib().mhd_set_metric_nodes(self.h, *[_p(a.ravel(order="F")) for a in xs]), "mhd_set_metric_nodes")
 
     def close(self):
         if getattr(self, "h", None):
@@ -263,3 +280,46 @@ def reconstruct(solver, f, lshape, direction):
     _ck(lib().mhd_reconstruct(solver.h, direction, _p(f), f.shape[0], f.shape[1], f.shape[2], lshape[0], lshape[1],
                               lshape[2], _p(fl), _p(fr)), "mhd_reconstruct")
     return fl, fr
+
+
+def metric_nodes_host(grid):
+    """the general metric's arrays of a node grid as the library builds
+    them (host code, no device needed): dict with nf[f] (3, face shape),
+    af[f], iaf[f], ef[e] (owned edge shape), ivol, cm (18, cell shape),
+    nc (9, cell shape), ih (3, cell shape)"""
+    nx0, ny0, nz0, ng = grid.nx0, grid.ny0, grid.nz0, grid.ng
+    X, Y, Z = (np.asfortranarray(a, dtype=np.float64).ravel(order="F") for a in grid.nodes)
+    n = lib().mhd_metric_nodes_host(nx0, ny0, nz0, ng, _p(X), _p(Y), _p(Z), None)
+    if n < 0:
+        raise MhdError("mhd_metric_nodes_host: %s" % _errstr(-n))
+    out = np.zeros(n)
+    r = lib().mhd_metric_nodes_host(nx0, ny0, nz0, ng, _p(X), _p(Y), _p(Z), _p(out))
+    if r < 0:
+        raise MhdError("mhd_metric_nodes_host: %s" % _errstr(-r))
+    nx, ny, nz = grid.nx, grid.ny, grid.nz
+    fshape = [(nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1)]
+    eshape = [(nx0, ny0 + 1, nz0 + 1), (nx0 + 1, ny0, nz0 + 1), (nx0 + 1, ny0 + 1, nz0)]
+    cs = (nx, ny, nz)
+    pos = [0]
+
+    def take(shape, comps=None):
+        cnt = int(np.prod(shape)) * (comps or 1)
+        a = out[pos[0]:pos[0] + cnt]
+        pos[0] += cnt
+        if comps:
+            return a.reshape((comps,) + tuple(shape[::-1])).transpose((0, 3, 2, 1))
+        return a.reshape(shape, order="F")
+
+    res = dict(nf=[], af=[], iaf=[], ef=[])
+    for f in range(3):
+        res["nf"].append(take(fshape[f], 3))
+        res["af"].append(take(fshape[f]))
+        res["iaf"].append(take(fshape[f]))
+    for e in range(3):
+        res["ef"].append(take(eshape[e]))
+    res["ivol"] = take(cs)
+    res["cm"] = take(cs, 18)
+    res["nc"] = take(cs, 9)
+    res["ih"] = take(cs, 3)
+    assert pos[0] == n
+    return res
