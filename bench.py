@@ -150,6 +150,56 @@ def max_divb(st, g):
     return float(np.abs(div).max())
 
 
+SHELL_SHAPE = (96, 64, 128)
+
+
+def shell_config(steps, warmup=2):
+    """spherical-shell dipole-plus-inflow set-up: 96 (r, log-stretched 3-30)
+    x 64 (theta) x 128 (phi), WENO5, CT, RK2, binary64 against fp32 stencils
+    with binary64 accumulation; the general metric built from the node
+    coordinates.  Fixed dt = 0.5 x the CFL-0.3 dt of the initial state (the
+    inflow compresses the dayside, so the initial dt alone is not stable for
+    the whole run).  Returns the record of both runs."""
+    from mhd3d_mpi_fortran_amd import shell
+    nr, nth, nph = SHELL_SHAPE
+    g = shell.spherical_shell(nr, nth, nph, ng=3)
+    st = shell.dipole_inflow(g, seed=1)
+    geo = g.geometry()
+    bmax = max(float(np.abs(st[k]).max()) for k in ("bi", "bj", "bk"))
+    hmin = float(np.cbrt(geo["vol"].min()))
+    out, states, tau_used = {}, {}, None
+    for prec in ("fp64", "mixed"):
+        p = gpu.make_params(nr, nth, nph, ng=3, reconop=3, stepop=2, bc=("even", "even", "periodic"),
+                            precision=prec)
+        s = gpu.Solver(p, g)
+        try:
+            s.upload(st)
+            s.initialize(0.)
+            if tau_used is None:
+                tau_used = .5 * s.get_dt()
+            s.step(warmup, tau_used)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.step(steps, tau_used)
+            s.synchronize()
+            wall = time.perf_counter() - t0
+            flags = s.check_state()
+            states[prec] = s.download()
+        finally:
+            s.close()
+        d = shell.div_b(g, states[prec])[g.ng:-g.ng, g.ng:-g.ng, g.ng:-g.ng]
+        out[prec] = {"value": nr * nth * nph * steps / wall, "ms_per_step": wall / steps * 1e3,
+                     "state_flags": flags, "max_divB_normalised": float(np.abs(d).max()) * hmin / bmax}
+    ra, rb = g.interior(states["mixed"]["rho"]), g.interior(states["fp64"]["rho"])
+    return {"workload": "spherical shell %dx%dx%d (r log 3-30, theta .3-pi-.3, phi periodic), dipole + "
+                        "inflow v=4 along -x, Bz=-0.1, 1%% density noise; WENO5, CT, RK2, %d steps (+%d warm-up), "
+                        "dt %.4e fixed" % (nr, nth, nph, steps, warmup, tau_used),
+            "fp64": out["fp64"], "mixed": out["mixed"],
+            "speedup_mixed_vs_fp64": out["mixed"]["value"] / out["fp64"]["value"],
+            "density_L1_mixed_vs_fp64": float(np.abs(ra - rb).sum() / np.abs(rb).sum()),
+            "initial_max_divB_normalised": float(np.abs(shell.div_b(g, st)).max()) * hmin / bmax}
+
+
 def host_cores():
     try:
         n = len(os.sched_getaffinity(0))
@@ -227,11 +277,17 @@ def main():
     ap.add_argument("--train-n", type=int, default=128)
     ap.add_argument("--train-epochs", type=int, default=2)
     ap.add_argument("--theta", type=float, default=None, help="smoothness threshold of the learned reconstruction")
+    ap.add_argument("--shell-steps", type=int, default=200,
+                    help="steps of the spherical-shell side record (0: skip it)")
+    ap.add_argument("--shell-only", action="store_true", help="print only the spherical-shell record")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the MPI decomposition of the reference is replaced by a single "
               "whole-domain block; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.shell_only:
+        print(json.dumps(shell_config(args.shell_steps)), flush=True)
+        return
     n = args.n
     if args.theta is not None:
         sg.THETA = args.theta
@@ -324,6 +380,9 @@ def main():
                             "mlp_flops_per_face": MLP_FLOPS,
                             "mlp_tflops_in_flux_kernels": faces_per_step * MLP_FLOPS / (fl_ms * 1e-3) / 1e12,
                             "bf16_peak_tflops": BF16_PEAK_TFLOPS}
+    if args.shell_steps > 0:
+        log("spherical-shell record (%d steps) ..." % args.shell_steps)
+        out["shell_config"] = shell_config(args.shell_steps)
     if not args.no_cpu_baseline:
         cores = host_cores()
         ranks = 16 if cores >= 16 else 8

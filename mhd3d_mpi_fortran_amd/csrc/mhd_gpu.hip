@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -90,6 +91,8 @@ int mhd_create(mhd_ctx** out, const mhd_params* p) {
     c->ne[1] = (long)(d.nx0 + 1) * d.ny0 * (d.nz0 + 1);
     c->ne[2] = (long)(d.nx0 + 1) * (d.ny0 + 1) * d.nz0;
     c->ninterior = (long)d.nx0 * d.ny0 * d.nz0;
+    c->emf_fill = kEmfFillThreads;
+    if (const char* ev = getenv("MHD_EMF_FILL_THREADS")) c->emf_fill = atol(ev);
     long nmax = c->ncell;
     nmax = c->nbi > nmax ? c->nbi : nmax;
     nmax = c->nbj > nmax ? c->nbj : nmax;

@@ -37,6 +37,11 @@ enum { SC_TAU = 0, SC_TAU0 = 1, SC_DTCFL = 2 };
 
 struct SurrogateState;
 
+// EMF launches whose long-run grid would leave the chip short of waves
+// (fewer than ~8 per SIMD: 256 CUs x 4 SIMDs x 8 waves x 64 lanes) take
+// short runs
+constexpr long kEmfFillThreads = 256L * 4 * 8 * 64;
+
 struct mhd_ctx {
     mhd_params p{};
     mhd::Dims d{};
@@ -46,6 +51,9 @@ struct mhd_ctx {
     bool have_geometry = false;
     long ncell = 0, nbi = 0, nbj = 0, nbk = 0, ninterior = 0, ndtpart = 0, nmax = 0;
     long nf[3] = {0, 0, 0}, ne[3] = {0, 0, 0};
+    // EMF launches with fewer long-run threads than this take short runs
+    // (environment MHD_EMF_FILL_THREADS overrides it at mhd_create)
+    long emf_fill = 0;
     // state (reference layout)
     double *rho = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr, *pr = nullptr;
     double *bx = nullptr, *by = nullptr, *bz = nullptr, *vf = nullptr;

@@ -38,11 +38,26 @@ void flux_launch(mhd_ctx* c, Src<TI> s) {
     }
 }
 
-// edges handled per thread along the march axis
+// edges handled per thread along the march axis: long runs share the
+// sliding stencil windows; a launch whose long-run grid has fewer threads
+// than c->emf_fill takes runs of 2 instead, trading window reuse for latency
+// hiding
 template <int M>
 struct EmfSteps {
     static constexpr int T = M == TVD ? 16 : 8;
+    static constexpr int T_SHORT = 2;
 };
+
+template <typename R, int M, int K, int T, typename TI, typename TF>
+void emf_launch_kt(mhd_ctx* c, EmfIn<TI> in) {
+    const Dims& d = c->d;
+    const long ii = d.nx0 + (K != 1);
+    const long no = K == 3 ? d.nz0 : (K == 1 ? d.ny0 + 1 : d.ny0);
+    const long nm = K == 3 ? d.ny0 + 1 : d.nz0 + 1;
+    const long n = ii * no * ((nm + T - 1) / T);
+    launch(c, MHD_K_EMF, blocks(n), dim3(kBlock), k_emf_march<R, M, K, T, TI, TF>, d, make_coef<R>(c), in,
+           R(c->met.ef[K - 1]), c->met.mode == 2 ? c->gm.ef[K - 1] : (const double*)nullptr, (TF*)c->emf[K - 1], n);
+}
 
 template <typename R, int M, int K, typename TI, typename TF>
 void emf_launch_k(mhd_ctx* c, EmfIn<TI> in) {
@@ -51,9 +66,8 @@ void emf_launch_k(mhd_ctx* c, EmfIn<TI> in) {
     const long ii = d.nx0 + (K != 1);
     const long no = K == 3 ? d.nz0 : (K == 1 ? d.ny0 + 1 : d.ny0);
     const long nm = K == 3 ? d.ny0 + 1 : d.nz0 + 1;
-    const long n = ii * no * ((nm + T - 1) / T);
-    launch(c, MHD_K_EMF, blocks(n), dim3(kBlock), k_emf_march<R, M, K, T, TI, TF>, d, make_coef<R>(c), in,
-           R(c->met.ef[K - 1]), c->met.mode == 2 ? c->gm.ef[K - 1] : (const double*)nullptr, (TF*)c->emf[K - 1], n);
+    if (ii * no * ((nm + T - 1) / T) >= c->emf_fill) emf_launch_kt<R, M, K, T, TI, TF>(c, in);
+    else emf_launch_kt<R, M, K, EmfSteps<M>::T_SHORT, TI, TF>(c, in);
 }
 
 // edge k: (v1, v2, b1, b2) = (vy, vz, bj, bk), (vz, vx, bk, bi), (vx, vy, bi, bj);
