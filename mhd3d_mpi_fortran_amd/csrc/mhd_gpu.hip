@@ -61,8 +61,11 @@ int mhd_create(mhd_ctx** out, const mhd_params* p) {
     if (p->ng < min_ghosts(p->reconop)) return MHD_ERR_PARAM;
     // the ghost fill copies from owned cells only: each axis needs nx0 >= ng
     if (p->nx0 < p->ng || p->ny0 < p->ng || p->nz0 < p->ng) return MHD_ERR_PARAM;
-    // index spaces (ghost-inclusive faces and edges) must stay below 2^31
-    if ((double)(p->nx0 + 2 * p->ng + 1) * (p->ny0 + 2 * p->ng + 1) * (p->nz0 + 2 * p->ng + 1) >= 2147483647.)
+    // index spaces (ghost-inclusive faces and edges) must stay below 2^29, so
+    // that every binary64 array stays below 2^32 bytes (32-bit byte offsets
+    // in the stencil kernels); about 800^3 cells, which is what the HBM of
+    // one GPU holds for the step's arrays
+    if ((double)(p->nx0 + 2 * p->ng + 1) * (p->ny0 + 2 * p->ng + 1) * (p->nz0 + 2 * p->ng + 1) >= 536870912.)
         return MHD_ERR_PARAM;
     for (int a = 0; a < 3; ++a)
         if (p->bc[a] != MHD_BC_EVEN && p->bc[a] != MHD_BC_PERIODIC) return MHD_ERR_PARAM;
@@ -639,7 +642,8 @@ static void do_bxyz(mhd_ctx* c, bool want_vf, bool want_shadow, bool want_dt, bo
     // face-normal velocities where the EMF will read them: binary64 with vf,
     // fp32 with the stencil copies
     const BMet m = bmet(c, want_vf && !mixed(c), want_shadow && mixed(c));
-    launch(c, MHD_K_BXYZ, blocks(c->nmax), dim3(kBlock), k_bxyz, c->d, c->p.gam, c->p.cfl,
+    auto kb = c->met.mode == 2 ? k_bxyz<true> : k_bxyz<false>;
+    launch(c, MHD_K_BXYZ, blocks(c->nmax), dim3(kBlock), kb, c->d, c->p.gam, c->p.cfl,
            CFaces{c->bi, c->bj, c->bk}, CCells{c->rho, c->vx, c->vy, c->vz, c->pr},
            store_cellb ? c->bx : (double*)nullptr, c->by, c->bz, want_vf ? c->vf : (double*)nullptr, sh, m,
            sdx(c), sdy(c), sdz(c), want_dt ? c->dtpart : (double*)nullptr, c->ncell, c->nbi, c->nbj, c->nbk);
@@ -675,7 +679,9 @@ static void do_update_t(mhd_ctx* c, double* const* out) {
     Cells so{out[0], out[1], out[2], out[3], out[4]};
     PrevRK pv{c->rhop, c->mxp, c->myp, c->mzp, c->engp};
     FluxSet<TF> F{(const TF*)c->flux[0], (const TF*)c->flux[1], (const TF*)c->flux[2], c->nf[0], c->nf[1], c->nf[2]};
-    launch(c, MHD_K_UPDATE, blocks(c->ninterior), dim3(kBlock), k_update<NST, ST, TF>, c->d, c->p.gam - 1.,
+    const bool gen = c->met.mode == 2;
+    auto ku = gen ? k_update<NST, ST, TF, true> : k_update<NST, ST, TF, false>;
+    launch(c, MHD_K_UPDATE, blocks(c->ninterior), dim3(kBlock), ku, c->d, c->p.gam - 1.,
            (const double*)(c->scal + SC_TAU), s, so, pv, F, sdx(c), sdy(c), sdz(c), gmet(c), c->ninterior);
     CFaces b{c->bi, c->bj, c->bk};
     Faces bo{out[8], out[9], out[10]};
@@ -683,7 +689,8 @@ static void do_update_t(mhd_ctx* c, double* const* out) {
     const long nct = (long)(c->d.nx0 + 1) * c->d.ny0 * c->d.nz0 + (long)c->d.nx0 * (c->d.ny0 + 1) * c->d.nz0 +
                      (long)c->d.nx0 * c->d.ny0 * (c->d.nz0 + 1);
     const CtFactor ctf{{c->met.ct[0], c->met.ct[1], c->met.ct[2]}};
-    launch(c, MHD_K_CT, blocks(nct), dim3(kBlock), k_ct<NST, ST, TF>, c->d, (const double*)(c->scal + SC_TAU), ctf, b,
+    auto kc = gen ? k_ct<NST, ST, TF, true> : k_ct<NST, ST, TF, false>;
+    launch(c, MHD_K_CT, blocks(nct), dim3(kBlock), kc, c->d, (const double*)(c->scal + SC_TAU), ctf, b,
            bo, bp, (const TF*)c->emf[0], (const TF*)c->emf[1], (const TF*)c->emf[2], (const double*)c->dx,
            (const double*)c->dy, (const double*)c->dz, gmet(c));
 }

@@ -82,8 +82,8 @@ __device__ __forceinline__ void face_flux(const Coef<R>& c, const R* l, const R*
     R v0, v1;
     // left state, positive half-moments
     const R unl = D == 1 ? ul : (D == 2 ? vl : wl);
-    R leng = R(.5) * rl * (sq(ul) + sq(vl) + sq(wl)) + pl / c.gm1;
-    R lam = rl / (R(2) * pl);
+    R leng = R(.5) * rl * (sq(ul) + sq(vl) + sq(wl)) + qdiv(pl, c.gm1);
+    R lam = qdiv(rl, R(2) * pl);
     vface<R>(lam, unl, +1, c.rspi, v0, v1);
     R fr0 = rl * v1, fmx = rl * ul * v1, fmy = rl * vl * v1, fmz = rl * wl * v1;
     R fe = (leng + R(.5) * pl) * v1;
@@ -92,7 +92,7 @@ __device__ __forceinline__ void face_flux(const Coef<R>& c, const R* l, const R*
     if (D == 3) fmz = fmz + pl * v0;
     fe = fe + R(.5) * pl * unl * v0;
     R bsq = sq(bxl) + sq(byl) + sq(bzl);
-    lam = rl / (R(2) * pl + bsq);
+    lam = qdiv(rl, R(2) * pl + bsq);
     vface<R>(lam, unl, +1, c.rspi, v0, v1);
     const R bnl = D == 1 ? bxl : (D == 2 ? byl : bzl);
     R sx = (D == 1 ? (R(.5) * bsq - bxl * bxl) : (-bxl * bnl)) * v0;
@@ -101,8 +101,8 @@ __device__ __forceinline__ void face_flux(const Coef<R>& c, const R* l, const R*
 
     // right state, negative half-moments
     const R unr = D == 1 ? ur : (D == 2 ? vr : wr);
-    leng = R(.5) * rr * (sq(ur) + sq(vr) + sq(wr)) + pr / c.gm1;
-    lam = rr / (R(2) * pr);
+    leng = R(.5) * rr * (sq(ur) + sq(vr) + sq(wr)) + qdiv(pr, c.gm1);
+    lam = qdiv(rr, R(2) * pr);
     vface<R>(lam, unr, -1, c.rspi, v0, v1);
     R gr0 = rr * v1, gmx = rr * ur * v1, gmy = rr * vr * v1, gmz = rr * wr * v1;
     R ge = (leng + R(.5) * pr) * v1;
@@ -111,7 +111,7 @@ __device__ __forceinline__ void face_flux(const Coef<R>& c, const R* l, const R*
     if (D == 3) gmz = gmz + pr * v0;
     ge = ge + R(.5) * pr * unr * v0;
     bsq = sq(bxr) + sq(byr) + sq(bzr);
-    lam = rr / (R(2) * pr + bsq);
+    lam = qdiv(rr, R(2) * pr + bsq);
     vface<R>(lam, unr, -1, c.rspi, v0, v1);
     const R bnr = D == 1 ? bxr : (D == 2 ? byr : bzr);
     R tx = (D == 1 ? (R(.5) * bsq - bxr * bxr) : (-bxr * bnr)) * v0;
@@ -166,8 +166,8 @@ __device__ __forceinline__ void face_flux_met(const Coef<R>& c, const R* l, cons
     {
         const R rl = l[0], ul = l[1], vl = l[2], wl = l[3], pl = l[4], bxl = l[5], byl = l[6], bzl = l[7];
         const R un = n[0] * ul + n[1] * vl + n[2] * wl;
-        const R leng = R(.5) * rl * (sq(ul) + sq(vl) + sq(wl)) + pl / c.gm1;
-        R lam = rl / (R(2) * pl);
+        const R leng = R(.5) * rl * (sq(ul) + sq(vl) + sq(wl)) + qdiv(pl, c.gm1);
+        R lam = qdiv(rl, R(2) * pl);
         vface<R>(lam, un, +1, c.rspi, v0, v1);
         fx[0] = rl * v1;
         fx[1] = rl * ul * v1 + pl * n[0] * v0;
@@ -175,7 +175,7 @@ __device__ __forceinline__ void face_flux_met(const Coef<R>& c, const R* l, cons
         fx[3] = rl * wl * v1 + pl * n[2] * v0;
         fx[4] = (leng + R(.5) * pl) * v1 + R(.5) * pl * un * v0;
         const R bsq = sq(bxl) + sq(byl) + sq(bzl);
-        lam = rl / (R(2) * pl + bsq);
+        lam = qdiv(rl, R(2) * pl + bsq);
         vface<R>(lam, un, +1, c.rspi, v0, v1);
         const R bn = n[0] * bxl + n[1] * byl + n[2] * bzl;
         fx[5] = (R(.5) * bsq * n[0] - bxl * bn) * v0;
@@ -186,8 +186,8 @@ __device__ __forceinline__ void face_flux_met(const Coef<R>& c, const R* l, cons
     {
         const R rr = r[0], ur = r[1], vr = r[2], wr = r[3], pr = r[4], bxr = r[5], byr = r[6], bzr = r[7];
         const R un = n[0] * ur + n[1] * vr + n[2] * wr;
-        const R leng = R(.5) * rr * (sq(ur) + sq(vr) + sq(wr)) + pr / c.gm1;
-        R lam = rr / (R(2) * pr);
+        const R leng = R(.5) * rr * (sq(ur) + sq(vr) + sq(wr)) + qdiv(pr, c.gm1);
+        R lam = qdiv(rr, R(2) * pr);
         vface<R>(lam, un, -1, c.rspi, v0, v1);
         fx[0] += rr * v1;
         fx[1] += rr * ur * v1 + pr * n[0] * v0;
@@ -195,7 +195,7 @@ __device__ __forceinline__ void face_flux_met(const Coef<R>& c, const R* l, cons
         fx[3] += rr * wr * v1 + pr * n[2] * v0;
         fx[4] += (leng + R(.5) * pr) * v1 + R(.5) * pr * un * v0;
         const R bsq = sq(bxr) + sq(byr) + sq(bzr);
-        lam = rr / (R(2) * pr + bsq);
+        lam = qdiv(rr, R(2) * pr + bsq);
         vface<R>(lam, un, -1, c.rspi, v0, v1);
         const R bn = n[0] * bxr + n[1] * byr + n[2] * bzr;
         fx[5] += (R(.5) * bsq * n[0] - bxr * bn) * v0;
@@ -233,14 +233,28 @@ __global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<TI> q, 
     const ptrdiff_t base = ix(d.nx, d.ny, d.ng + i, d.ng + j, d.ng + k);
     const ptrdiff_t st = D == 1 ? 1 : (D == 2 ? (ptrdiff_t)d.nx : (ptrdiff_t)d.nx * d.ny);
     R l[8], r[8];
-    recon_line<R, M>(q.rho, base, st, c, l[0], r[0]);
-    recon_line<R, M>(q.vx, base, st, c, l[1], r[1]);
-    recon_line<R, M>(q.vy, base, st, c, l[2], r[2]);
-    recon_line<R, M>(q.vz, base, st, c, l[3], r[3]);
-    recon_line<R, M>(q.p, base, st, c, l[4], r[4]);
-    recon_line<R, M>(q.bx, base, st, c, l[5], r[5]);
-    recon_line<R, M>(q.by, base, st, c, l[6], r[6]);
-    recon_line<R, M>(q.bz, base, st, c, l[7], r[7]);
+    if (D == 1) {
+        // unit stride: the taps of one array are contiguous and load as
+        // wide vectors from one address
+        recon_line<R, M>(q.rho, base, st, c, l[0], r[0]);
+        recon_line<R, M>(q.vx, base, st, c, l[1], r[1]);
+        recon_line<R, M>(q.vy, base, st, c, l[2], r[2]);
+        recon_line<R, M>(q.vz, base, st, c, l[3], r[3]);
+        recon_line<R, M>(q.p, base, st, c, l[4], r[4]);
+        recon_line<R, M>(q.bx, base, st, c, l[5], r[5]);
+        recon_line<R, M>(q.by, base, st, c, l[6], r[6]);
+        recon_line<R, M>(q.bz, base, st, c, l[7], r[7]);
+    } else {
+        const Taps<R, M, TI> tp(base, st);
+        recon_taps<R, M>(q.rho, tp, c, l[0], r[0]);
+        recon_taps<R, M>(q.vx, tp, c, l[1], r[1]);
+        recon_taps<R, M>(q.vy, tp, c, l[2], r[2]);
+        recon_taps<R, M>(q.vz, tp, c, l[3], r[3]);
+        recon_taps<R, M>(q.p, tp, c, l[4], r[4]);
+        recon_taps<R, M>(q.bx, tp, c, l[5], r[5]);
+        recon_taps<R, M>(q.by, tp, c, l[6], r[6]);
+        recon_taps<R, M>(q.bz, tp, c, l[7], r[7]);
+    }
     if (MET == 2) {
         R n[3];
         gen_normal<R, D>(d, gm, i, j, k, n);

@@ -25,6 +25,19 @@ struct Coef {
 template <typename R>
 __device__ __forceinline__ R sq(R x) { return x * x; }
 
+// quotient and square root of the flux formulas: IEEE in binary64 (the
+// reference's arithmetic); in fp32 (mixed precision) the hardware
+// reciprocal and square root (1 ulp), which skip the range-scaling
+// sequence of the correctly rounded forms
+template <typename R>
+__device__ __forceinline__ R qdiv(R a, R b) { return a / b; }
+template <>
+__device__ __forceinline__ float qdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <typename R>
+__device__ __forceinline__ R qsqrt(R x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float qsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // SIGN(1.,x): IEEE sign bit, -0.0 gives -1
 template <typename R>
 __device__ __forceinline__ R sgn1(R x) { return copysign(R(1), x); }
@@ -173,6 +186,37 @@ __device__ __forceinline__ void recon_line(const T* __restrict__ f, ptrdiff_t ba
     S::run(s, c, fl, fr);
 }
 
+// 32-bit byte offsets of a stencil's taps around base along stride, shared by
+// every array of one element type and layout: loads then take the array's
+// uniform base address plus one offset register per tap (mhd_create keeps
+// every ghost-inclusive array below 2^32 bytes)
+template <typename R, int M, typename T>
+struct Taps {
+    using S = Stencil<R, M>;
+    unsigned o[S::n];
+    __device__ __forceinline__ Taps(ptrdiff_t base, ptrdiff_t stride) {
+        const unsigned b = (unsigned)base * (unsigned)sizeof(T), s = (unsigned)stride * (unsigned)sizeof(T);
+#pragma unroll
+        for (int m = 0; m < S::n; ++m) o[m] = b + (unsigned)(S::lo + m) * s;
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ T load_at(const T* __restrict__ f, unsigned byte_off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(f) + byte_off);
+}
+
+// recon_line with precomputed tap offsets
+template <typename R, int M, typename T>
+__device__ __forceinline__ void recon_taps(const T* __restrict__ f, const Taps<R, M, T>& tp, const Coef<R>& c,
+                                           R& fl, R& fr) {
+    using S = Stencil<R, M>;
+    R s[S::n];
+#pragma unroll
+    for (int m = 0; m < S::n; ++m) s[m] = R(load_at(f, tp.o[m]));
+    S::run(s, c, fl, fr);
+}
+
 // get_v_face (mhd3d3.f90:2594-2615): the side's half-moments.
 //   side = +1: v0p = .5 erfc(-a v), v1p = v v0p + b
 //   side = -1: v0n = .5 erfc( a v), v1n = v v0n - b
@@ -200,7 +244,7 @@ __device__ __forceinline__ void vface<double>(double lam, double v, int side, do
 // serves both.  erfc(-u) = 2 - erfc(u).
 template <>
 __device__ __forceinline__ void vface<float>(float lam, float v, int side, float rspi, float& v0, float& v1) {
-    const float a = sqrtf(lam);
+    const float a = __builtin_amdgcn_sqrtf(lam);
     const float x = a * v;
     const float e = __expf(-(x * x));
     const float b = .5f * e * __builtin_amdgcn_rcpf(a * rspi);

@@ -30,8 +30,9 @@ struct FluxSet {
     long nfx, nfy, nfz;
 };
 
-// gm.ivol != null: general metric, div = (sum_f F_f |S_f| out) / V
-template <int NST, int ST, typename TF>
+// GEN: general metric (gm), div = (sum_f F_f |S_f| out) / V; a template
+// parameter so that the other grids' kernel carries none of its registers
+template <int NST, int ST, typename TF, bool GEN>
 __global__ void __launch_bounds__(kBlock) k_update(Dims d, double gm1, const double* __restrict__ tau_p, CCells s,
                                                   Cells so, PrevRK pv, FluxSet<TF> F, const double* __restrict__ dx,
                                                   const double* __restrict__ dy, const double* __restrict__ dz,
@@ -48,7 +49,7 @@ __global__ void __launch_bounds__(kBlock) k_update(Dims d, double gm1, const dou
     const ptrdiff_t fy0 = ix(d.nx0, d.ny0 + 1, i, j, k), fy1 = fy0 + d.nx0;
     const ptrdiff_t fz0 = ix(d.nx0, d.ny0, i, j, k), fz1 = fz0 + (ptrdiff_t)d.nx0 * d.ny0;
     double ar[6] = {0., 0., 0., 0., 0., 0.}, iv = 0.;
-    if (gm.ivol) {
+    if constexpr (GEN) {
         const ptrdiff_t gi = ix(d.nx + 1, d.ny, ng + i, ng + j, ng + k), gj = ix(d.nx, d.ny + 1, ng + i, ng + j, ng + k);
         ar[0] = gm.af[0][gi];
         ar[1] = gm.af[0][gi + 1];
@@ -59,7 +60,7 @@ __global__ void __launch_bounds__(kBlock) k_update(Dims d, double gm1, const dou
         iv = gm.ivol[q];
     }
     auto div = [&](int f) -> double {
-        if (gm.ivol)
+        if constexpr (GEN)
             return ((double(F.x[f * F.nfx + fx0 + 1]) * ar[1] - double(F.x[f * F.nfx + fx0]) * ar[0]) +
                     (double(F.y[f * F.nfy + fy1]) * ar[3] - double(F.y[f * F.nfy + fy0]) * ar[2]) +
                     (double(F.z[f * F.nfz + fz1]) * ar[5] - double(F.z[f * F.nfz + fz0]) * ar[4])) * iv;
@@ -138,9 +139,9 @@ struct CtFactor {
     double f[3];
 };
 
-// gm.ivol != null: general metric, the EMFs are E . dl and the face update
-// divides by the face's area
-template <int NST, int ST, typename TF>
+// GEN: general metric (gm), the EMFs are E . dl and the face update divides
+// by the face's area
+template <int NST, int ST, typename TF, bool GEN>
 __global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict__ tau_p, CtFactor ctf, CFaces b,
                                               Faces bo, CFaces bp, const TF* __restrict__ ei,
                                               const TF* __restrict__ ej, const TF* __restrict__ ek,
@@ -165,7 +166,7 @@ __global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict_
         const double dej = double(ej[e + (ptrdiff_t)(nx0 + 1) * ny0]) - double(ej[e]);
         q = ix(d.nx + 1, d.ny, ng + i, ng + j, ng + k);
         old = b.bi[q];
-        val = gm.ivol ? old - tau * gm.iaf[0][q] * (dek - dej)
+        val = GEN ? old - tau * gm.iaf[0][q] * (dek - dej)
                       : old - tau * ctf.f[0] * (dek / dy[ng + j - 1] - dej / dz[ng + k - 1]);
         out = bo.bi;
         prev = bp.bi;
@@ -177,7 +178,7 @@ __global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict_
         const double dek = double(ek[e + 1]) - double(ek[e]);
         q = ix(d.nx, d.ny + 1, ng + i, ng + j, ng + k);
         old = b.bj[q];
-        val = gm.ivol ? old - tau * gm.iaf[1][q] * (dei - dek)
+        val = GEN ? old - tau * gm.iaf[1][q] * (dei - dek)
                       : old - tau * ctf.f[1] * (dei / dz[ng + k - 1] - dek / dx[ng + i - 1]);
         out = bo.bj;
         prev = bp.bj;
@@ -189,7 +190,7 @@ __global__ void __launch_bounds__(kBlock) k_ct(Dims d, const double* __restrict_
         const double dei = double(ei[e + nx0]) - double(ei[e]);
         q = ix(d.nx, d.ny, ng + i, ng + j, ng + k);
         old = b.bk[q];
-        val = gm.ivol ? old - tau * gm.iaf[2][q] * (dej - dei)
+        val = GEN ? old - tau * gm.iaf[2][q] * (dej - dei)
                       : old - tau * ctf.f[2] * (dej / dx[ng + i - 1] - dei / dy[ng + j - 1]);
         out = bo.bk;
         prev = bp.bk;
@@ -348,6 +349,8 @@ __device__ __forceinline__ double cfl_dt(double gam, double cfl, double u, doubl
     return cfl / (vcfl / dxi + vcfl / dyj + vcfl / dzk);
 }
 
+// GEN: general metric (met.g)
+template <bool GEN>
 __global__ void __launch_bounds__(kBlock) k_bxyz(Dims d, double gam, double cfl, CFaces b, CCells s,
                                                 double* __restrict__ bx, double* __restrict__ by,
                                                 double* __restrict__ bz, double* __restrict__ vf, Shadow sh, BMet met,
@@ -366,7 +369,7 @@ __global__ void __launch_bounds__(kBlock) k_bxyz(Dims d, double gam, double cfl,
         double y = .5 * (bj1 + bj0);
         double z = .5 * (bk1 + bk0);
         const double u = s.vx[t], v = s.vy[t], w = s.vz[t], r = s.rho[t], pp = s.p[t];
-        if (met.g.ivol) {
+        if constexpr (GEN) {
             const double beta[6] = {bi0, bi1, bj0, bj1, bk0, bk1};
             double B[3];
             gen_cellb(met.g, t, beta, B);
@@ -414,7 +417,7 @@ __global__ void __launch_bounds__(kBlock) k_bxyz(Dims d, double gam, double cfl,
             }
         }
         if (part)
-            dt = met.g.ivol ? cfl_dt_gen(gam, cfl, u, v, w, x, y, z, r, pp, met.g, t)
+            dt = GEN ? cfl_dt_gen(gam, cfl, u, v, w, x, y, z, r, pp, met.g, t)
                             : cfl_dt(gam, cfl, u, v, w, x, y, z, r, pp, dx[i - 1], dy[j - 1], dz[k - 1]);
     }
     if (sh.bi) {

@@ -64,13 +64,21 @@ __device__ __forceinline__ void load5(const T* __restrict__ f, ptrdiff_t base, p
     for (int k = 0; k < 5; ++k) m = fmax(m, fabs(w[k] - c));
 }
 
+// centre c = (w1+w2)/2 and range m = max |w - c| of a loaded window
+__device__ __forceinline__ void window_stats(const float* w, float& c, float& m) {
+    c = .5f * (w[1] + w[2]);
+    m = 0.f;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) m = fmaxf(m, fabsf(w[k] - c));
+}
+
 // physical scale of each variable at the face from the window centres
 // c = (rho, vx, vy, vz, p, bx, by, bz)
 template <typename A>
 __device__ __forceinline__ void var_scales(const A* c, A* sc) {
     const A r = fabs(c[0]), p = fabs(c[4]);
-    const A v = fmax(fmax(fabs(c[1]), fabs(c[2])), fabs(c[3])) + sqrt(p / fmax(r, A(1e-30)));
-    const A b = fmax(fmax(fabs(c[5]), fabs(c[6])), fabs(c[7])) + sqrt(p);
+    const A v = fmax(fmax(fabs(c[1]), fabs(c[2])), fabs(c[3])) + qsqrt(qdiv(p, fmax(r, A(1e-30))));
+    const A b = fmax(fmax(fabs(c[5]), fabs(c[6])), fabs(c[7])) + qsqrt(p);
     sc[0] = r;
     sc[1] = sc[2] = sc[3] = v;
     sc[4] = p;
@@ -487,43 +495,56 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
         const long t = ((long)xcd_block<kXcdRun / SUR_GROUPS>() * SUR_GROUPS + grp) * blockDim.x + threadIdx.x;
         const bool live = t < nface;
         if (__ballot(live) == 0ull) break;  // wave-uniform: later groups are past the end too
-        int i = 1, j = 1, k = 1;
-        if (live) unflat(t, ii, jj, i, j, k);
-        const ptrdiff_t base = ix(d.nx, d.ny, d.ng + i, d.ng + j, d.ng + k);
-        float x[NI], cc[8], mm[8], sc[8], rough = 0.f;
+        // lanes past the end read the first face's windows (their results
+        // are never written), so the loads need no branches
+        int i, j, k;
+        unflat(live ? t : 0, ii, jj, i, j, k);
+        // 32-bit byte offsets of the 5 taps from each array's uniform base
+        // (every array stays below 2^32 bytes, checked at mhd_create)
+        const unsigned base = (unsigned)ix(d.nx, d.ny, d.ng + i, d.ng + j, d.ng + k);
+        unsigned off[5];
+#pragma unroll
+        for (int a = 0; a < 5; ++a) off[a] = (base + (unsigned)((a - 2) * st)) * 4u;
+        // windows in physical units
+        float x[NI], cc[8], mm[8], sc[8];
 #pragma unroll
         for (int v = 0; v < 8; ++v) {
-            if (live) load5<float>(f[v], base, st, x + 5 * v, cc[v], mm[v]);
-            else {
+            if constexpr (D == 1) {  // contiguous taps: wide loads from one address
 #pragma unroll
-                for (int a = 0; a < 5; ++a) x[5 * v + a] = 0.f;
-                cc[v] = mm[v] = 0.f;
+                for (int a = 0; a < 5; ++a) x[5 * v + a] = f[v][(ptrdiff_t)base + a - 2];
+            } else {
+#pragma unroll
+                for (int a = 0; a < 5; ++a) x[5 * v + a] = load_at(f[v], off[a]);
             }
+            window_stats(x + 5 * v, cc[v], mm[v]);
         }
         var_scales<float>(cc, sc);
-        // per variable, in physical units: the linear face value A, the gate
-        // times the window scale G, and the clamp range [lo, hi] of the two
-        // cells next to the face (PDM's own bound, mhd3d3.f90:1142-1166)
-        float A[8], G[8], lo[8], hi[8];
+        // per variable, in physical units: the linear face value A = lin of
+        // the window (the 4th-order interpolation commutes with the affine
+        // normalisation: its weights sum to 1), the gate times the window
+        // scale G = the largest |second difference| of the window, and the
+        // clamp range [lo, hi] of the two cells next to the face (PDM's own
+        // bound, mhd3d3.f90:1142-1166).  The smoothness test rough <= theta
+        // on the normalised window is G <= theta * s; the normalisation
+        // itself is only needed as the MLP's input.
+        float A[8], G[8], lo[8], hi[8], ss[8];
+        bool smooth = true, curved = false;
 #pragma unroll
         for (int v = 0; v < 8; ++v) {
-            const float ss = norm_scale<float>(mm[v], sc[v]);
+            ss[v] = norm_scale<float>(mm[v], sc[v]);
             const float w1 = x[5 * v + 1], w2 = x[5 * v + 2];
             lo[v] = fminf(w1, w2);
             hi[v] = fmaxf(w1, w2);
-            const float r = __frcp_rn(ss);
-#pragma unroll
-            for (int a = 0; a < 5; ++a) x[5 * v + a] = (x[5 * v + a] - cc[v]) * r;
-            const float gate = rough5(x + 5 * v);
-            rough = fmaxf(rough, gate);
-            A[v] = cc[v] + ss * lin4(x + 5 * v);
-            G[v] = ss * gate;
+            G[v] = rough5(x + 5 * v);
+            A[v] = lin4(x + 5 * v);
+            smooth = smooth && (G[v] <= theta * ss[v]);
+            curved = curved || (G[v] > 0.f);
         }
         // the MLP only matters on faces that use the net and have a non-zero
         // gate (all gates are zero on constant or linear windows, where
         // y = lin exactly); MFMA needs the whole wave, so the test is per wave
         float y[NO];
-        const bool need = live && rough <= theta && rough > 0.f;
+        const bool need = live && smooth && curved;
         if (__ballot(need)) {
             // A, G and the clamp range wait in LDS (each thread its own
             // column) while the MLP holds the registers
@@ -533,6 +554,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
                 stash[1][v][threadIdx.x] = G[v];
                 stash[2][v][threadIdx.x] = lo[v];
                 stash[3][v][threadIdx.x] = hi[v];
+            }
+            // the net's inputs: windows centred on c and divided by s
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                const float r = __frcp_rn(ss[v]);
+#pragma unroll
+                for (int a = 0; a < 5; ++a) x[5 * v + a] = (x[5 * v + a] - cc[v]) * r;
             }
             mlp_core(W, x, lane, y);
 #pragma unroll
@@ -552,7 +580,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
             lv[v] = fminf(hi[v], fmaxf(lo[v], A[v] + G[v] * y[v]));
             rv[v] = fminf(hi[v], fmaxf(lo[v], A[v] + G[v] * y[8 + v]));
         }
-        const bool fb = !(rough <= theta) || !(lv[0] > 0.f) || !(rv[0] > 0.f) || !(lv[4] > 0.f) || !(rv[4] > 0.f);
+        const bool fb = !smooth || !(lv[0] > 0.f) || !(rv[0] > 0.f) || !(lv[4] > 0.f) || !(rv[4] > 0.f);
         // fallback faces go to a list that k_flux_fix evaluates with PDM
         const unsigned long long fm = __ballot(live && fb);
         if (fm) {
