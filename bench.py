@@ -63,9 +63,9 @@ MLP_FLOPS = 2 * (40 * 32 + 32 * 32 + 32 * 16)
 # HBM bytes per launch of the stencil kernels, from the rocprofv3 FETCH_SIZE /
 # WRITE_SIZE passes committed under profiles/ (same workload)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "blast256_sheared_traffic.json")
-TRAFFIC_KERNELS = {("flux", "mixed", True): ["sur::k_flux_sur<%d, true>" % d for d in (1, 2, 3)],
-                   ("flux", "mixed", False): ["mhd::k_flux<float, 1, %d, float, float, true>" % d for d in (1, 2, 3)],
-                   ("flux", "fp64", False): ["mhd::k_flux<double, 1, %d, double, double, true>" % d for d in (1, 2, 3)],
+TRAFFIC_KERNELS = {("flux", "mixed", True): ["sur::k_flux_sur<%d, 1>" % d for d in (1, 2, 3)],
+                   ("flux", "mixed", False): ["mhd::k_flux<float, 1, %d, float, float, 1>" % d for d in (1, 2, 3)],
+                   ("flux", "fp64", False): ["mhd::k_flux<double, 1, %d, double, double, 1>" % d for d in (1, 2, 3)],
                    ("emf", "mixed", True): ["mhd::k_emf_march<float, 1, %d, 8, float, float>" % d for d in (1, 2, 3)],
                    ("emf", "mixed", False): ["mhd::k_emf_march<float, 1, %d, 8, float, float>" % d for d in (1, 2, 3)],
                    ("emf", "fp64", False): ["mhd::k_emf_march<double, 1, %d, 8, double, double>" % d
@@ -97,15 +97,21 @@ def workload(n, sheared=True):
     return g, st
 
 
-def run_gpu(g, st, n, precision, weights, warmup, steps):
+def run_gpu(g, st, n, precision, weights, warmup, steps, breakdown=True):
+    """warm-up, then the timed steps with HIP events around the flux launches
+    only (the roofline's kernel: events around every launch of a step cost
+    the stream time and distort the kernels between them); then, for the
+    per-class breakdown, the same steps replayed from the same initial state
+    once per kernel class, each replay timing that class alone"""
     s = gpu.Solver(gpu.make_params(n, n, n, ng=4, reconop=1, stepop=1, bc="periodic", precision=precision,
                                    surrogate=1 if weights is not None else 0), g)
     s.upload(st)
-    tau = s.initialize(1e-4)
+    tau0 = s.initialize(1e-4)
     sur = None
     if weights is not None:
         sg.set_weights(s, weights)
         sg.configure(s, sg.THETA, count=True)   # count fallbacks over the warm-up steps only
+    tau = tau0
     if warmup:
         tau, _ = s.step(warmup, tau)
     if weights is not None:
@@ -113,19 +119,35 @@ def run_gpu(g, st, n, precision, weights, warmup, steps):
         sur = dict(faces=faces, fallbacks=fb)
         sg.configure(s, sg.THETA, count=False)
     s.synchronize()
-    s.profile(True)
+    s.profile_classes(["flux"])
     t0 = time.perf_counter()
     tau, tcfl = s.step(steps, tau)
     s.synchronize()
     wall = time.perf_counter() - t0
-    prof = {k: s.profile_read(k) for k in gpu.KCLASS}
+    flux_timed = s.profile_read("flux")
     s.profile(False)
     flags = s.check_state()
     st_out = s.download()
+    prof = {}
+    if breakdown:
+        for k in gpu.KCLASS:
+            s.upload(st)
+            tau = s.initialize(1e-4)
+            if warmup:
+                tau, _ = s.step(warmup, tau)
+            s.synchronize()
+            before = s.profile_read(k)
+            s.profile_classes([k])
+            s.step(steps, tau)
+            s.synchronize()
+            after = s.profile_read(k)
+            s.profile(False)
+            prof[k] = (after[0] - before[0], after[1] - before[1])
     s.close()
     units = {"emf": [n * (n + 1) * (n + 1), (n + 1) * n * (n + 1), (n + 1) * (n + 1) * n],
              "flux": [(n + 1) * n * n, n * (n + 1) * n, n * n * (n + 1)]}
-    return dict(wall=wall, tau=tau, prof=prof, flags=flags, state=st_out, units=units, sur=sur)
+    return dict(wall=wall, tau=tau, prof=prof, flux_timed=flux_timed, flags=flags, state=st_out, units=units,
+                sur=sur)
 
 
 def l1_rel(a, b, g):
@@ -310,7 +332,7 @@ def main():
     res = {}
     if args.only in ("plain", "both"):
         log("plain fp64 run ...")
-        res["plain"] = run_gpu(g, st, n, "fp64", None, args.warmup, args.steps)
+        res["plain"] = run_gpu(g, st, n, "fp64", None, args.warmup, args.steps, breakdown=args.only == "plain")
     if args.only in ("fast", "both"):
         log("fast run (mixed precision%s) ..." % (" + learned reconstruction" if sur else ""))
         res["fast"] = run_gpu(g, st, n, "mixed", weights, args.warmup, args.steps)
@@ -323,10 +345,10 @@ def main():
     K = args.steps
     value = n ** 3 * K / head["wall"]
 
-    # dominant kernel of the headline run, timed by HIP events on the
-    # library's stream over the timed region
-    kcls = max(("emf", "flux"), key=lambda k: head["prof"][k][1])
-    launches, ms = head["prof"][kcls]
+    # dominant kernel of the headline run (the face flux), timed by HIP
+    # events on the library's stream over the timed region
+    kcls = "flux"
+    launches, ms = head["flux_timed"]
     avg_s = ms / launches / 1e3
     units = sum(head["units"][kcls]) / 3.
     alg = ALG_BYTES[kcls][prec] * units
@@ -374,7 +396,7 @@ def main():
     if sur and head_name == "fast":
         s = head["sur"]
         faces_per_step = sum(head["units"]["flux"])
-        fl_ms = head["prof"]["flux"][1] / K
+        fl_ms = head["flux_timed"][1] / K
         out["surrogate"] = {"theta": sg.THETA, "fallback_fraction": s["fallbacks"] / max(s["faces"], 1),
                             "faces_counted": s["faces"], "training": train_info,
                             "mlp_flops_per_face": MLP_FLOPS,

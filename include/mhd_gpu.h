@@ -165,6 +165,11 @@ enum { MHD_K_FLUX = 0, MHD_K_EMF = 1, MHD_K_UPDATE = 2, MHD_K_CT = 3, MHD_K_FILL
  * standalone inference) */
 int mhd_profile(mhd_ctx *ctx, int enable);
 int mhd_profile_read(mhd_ctx *ctx, int kclass, int *launches, double *ms);
+/* time only the classes whose bit (1 << MHD_K_*) is set in mask (0: off);
+ * each timed launch is bracketed by two event records, which cost the
+ * stream some time: a caller timing a whole step brackets only the class it
+ * needs */
+int mhd_profile_classes(mhd_ctx *ctx, unsigned mask);
 
 /* ---- learned face reconstruction (no counterpart in the reference) ------
  * A net y = lin(x) + gate(x) * MLP(x), MLP 40 -> 32 -> 32 -> 16 (activation

@@ -40,35 +40,47 @@ struct VavFoot<MPWENO5> {
 
 // register window w[a][b]: a = offset along the march axis, b = along the
 // other corner axis, both LO..LO+N-1 relative to the current edge's base cell
-template <typename R, int M, bool MARCH_D1>
+template <typename R, int M, bool MARCH_D1, bool UNIT = false>
 struct VavWin {
     static constexpr int N = VavFoot<M>::N, LO = VavFoot<M>::LO;
     R w[N][N];
     // value at offset o1 along d1 and o2 along d2
     __device__ __forceinline__ R cv(int o1, int o2) const { return MARCH_D1 ? w[o1 - LO][o2 - LO] : w[o2 - LO][o1 - LO]; }
 
+    // ob, osm, oso: byte offset of the base cell and byte strides along the
+    // march and the other axis (32-bit: every array stays below 2^32 bytes);
+    // UNIT: the other axis is x, each row of the window is contiguous and
+    // loads as wide vectors from one address
     template <typename TI>
-    __device__ __forceinline__ void load(const TI* __restrict__ lv, ptrdiff_t b, ptrdiff_t sm, ptrdiff_t so) {
+    __device__ __forceinline__ void row(const TI* __restrict__ lv, unsigned orow, unsigned oso, R* out) {
+        if constexpr (UNIT) {
+            const TI* p = reinterpret_cast<const TI*>(reinterpret_cast<const char*>(lv) + orow);
 #pragma unroll
-        for (int a = 0; a < N; ++a)
+            for (int q = 0; q < N; ++q) out[q] = R(p[q + LO]);
+        } else {
 #pragma unroll
-            for (int q = 0; q < N; ++q) w[a][q] = R(lv[b + (ptrdiff_t)(a + LO) * sm + (ptrdiff_t)(q + LO) * so]);
+            for (int q = 0; q < N; ++q) out[q] = R(load_at(lv, orow + (unsigned)(q + LO) * oso));
+        }
     }
-    // advance one cell along the march axis (b is the new base cell)
     template <typename TI>
-    __device__ __forceinline__ void shift(const TI* __restrict__ lv, ptrdiff_t b, ptrdiff_t sm, ptrdiff_t so) {
+    __device__ __forceinline__ void load(const TI* __restrict__ lv, unsigned ob, unsigned osm, unsigned oso) {
+#pragma unroll
+        for (int a = 0; a < N; ++a) row(lv, ob + (unsigned)(a + LO) * osm, oso, w[a]);
+    }
+    // advance one cell along the march axis (ob is the new base cell)
+    template <typename TI>
+    __device__ __forceinline__ void shift(const TI* __restrict__ lv, unsigned ob, unsigned osm, unsigned oso) {
 #pragma unroll
         for (int a = 0; a < N - 1; ++a)
 #pragma unroll
             for (int q = 0; q < N; ++q) w[a][q] = w[a + 1][q];
-#pragma unroll
-        for (int q = 0; q < N; ++q) w[N - 1][q] = R(lv[b + (ptrdiff_t)(N - 1 + LO) * sm + (ptrdiff_t)(q + LO) * so]);
+        row(lv, ob + (unsigned)(N - 1 + LO) * osm, oso, w[N - 1]);
     }
 };
 
 // PDM: the d1-interpolation of footprint column m2 (offset m2-4 along d2)
-template <typename R, bool MD1>
-__device__ __forceinline__ R pdm_col(const VavWin<R, PDM, MD1>& W, int m2) {
+template <typename R, bool MD1, bool U>
+__device__ __forceinline__ R pdm_col(const VavWin<R, PDM, MD1, U>& W, int m2) {
     R s[8];
 #pragma unroll
     for (int m1 = 0; m1 < 8; ++m1) s[m1] = W.cv(m1 - 4, m2 - 4);
@@ -77,8 +89,8 @@ __device__ __forceinline__ R pdm_col(const VavWin<R, PDM, MD1>& W, int m2) {
 
 // corner velocity of one edge from its window (mhd3d3.f90:1858-2116); for
 // PDM, g holds the 8 column interpolations (interp2_pdm's first sweep)
-template <typename R, int M, bool MD1>
-__device__ __forceinline__ R vav_win(const VavWin<R, M, MD1>& W, const Coef<R>& c, const R* g = nullptr) {
+template <typename R, int M, bool MD1, bool U>
+__device__ __forceinline__ R vav_win(const VavWin<R, M, MD1, U>& W, const Coef<R>& c, const R* g = nullptr) {
     R v00_0, v10_0, v01_0, v11_0, v00_1, v01_1, v10_1, v11_1;
     if constexpr (M == PDM) {
         const R vi = interp8(g);
@@ -130,32 +142,35 @@ __device__ __forceinline__ R vav_win(const VavWin<R, M, MD1>& W, const Coef<R>& 
 // one column, so seven of PDM's eight column interpolations carry over: they
 // ride along in a register window and only the new column is interpolated
 // (the same interp8 calls on the same values: bit-identical).
-template <typename R, int M, bool MD1, int T, typename TI>
+template <typename R, int M, bool MD1, bool UNIT, int T, typename TI>
 __device__ __forceinline__ void vav_march(const TI* __restrict__ lv, ptrdiff_t b, ptrdiff_t s1, ptrdiff_t s2,
                                           int cnt, const Coef<R>& c, R (&out)[T]) {
     const ptrdiff_t sm = MD1 ? s1 : s2, so = MD1 ? s2 : s1;
-    VavWin<R, M, MD1> W;
-    W.load(lv, b, sm, so);
+    constexpr unsigned E = sizeof(TI);
+    unsigned ob = (unsigned)b * E;
+    const unsigned osm = (unsigned)sm * E, oso = (unsigned)so * E;
+    VavWin<R, M, MD1, UNIT> W;
+    W.load(lv, ob, osm, oso);
     R g[8];
     if constexpr (M == PDM) {
 #pragma unroll
-        for (int m2 = 0; m2 < 8; ++m2) g[m2] = pdm_col<R, MD1>(W, m2);
+        for (int m2 = 0; m2 < 8; ++m2) g[m2] = pdm_col(W, m2);
     }
 #pragma unroll
     for (int s = 0; s < T; ++s) {
         if (s < cnt) {
-            out[s] = vav_win<R, M, MD1>(W, c, g);
+            out[s] = vav_win<R, M>(W, c, g);
             if (s + 1 < cnt) {
-                b += sm;
-                W.shift(lv, b, sm, so);
+                ob += osm;
+                W.shift(lv, ob, osm, oso);
                 if constexpr (M == PDM) {
                     if constexpr (MD1) {
 #pragma unroll
-                        for (int m2 = 0; m2 < 8; ++m2) g[m2] = pdm_col<R, MD1>(W, m2);
+                        for (int m2 = 0; m2 < 8; ++m2) g[m2] = pdm_col(W, m2);
                     } else {
 #pragma unroll
                         for (int m2 = 0; m2 < 7; ++m2) g[m2] = g[m2 + 1];
-                        g[7] = pdm_col<R, MD1>(W, 7);
+                        g[7] = pdm_col(W, 7);
                     }
                 }
             }
@@ -204,8 +219,9 @@ __global__ void __launch_bounds__(kBlock) k_emf_march(Dims d, Coef<R> c, EmfIn<T
     constexpr bool MD1 = EmfGeom<K>::MD1;
 
     R v1av[T], v2av[T];
-    vav_march<R, M, MD1, T>(in.v1, bc0, s1, s2, cnt, c, v1av);
-    vav_march<R, M, MD1, T>(in.v2, bc0, s1, s2, cnt, c, v2av);
+    // the window's other axis is x (unit stride) for K=2 and K=3
+    vav_march<R, M, MD1, K != 1, T>(in.v1, bc0, s1, s2, cnt, c, v1av);
+    vav_march<R, M, MD1, K != 1, T>(in.v2, bc0, s1, s2, cnt, c, v2av);
 
     // face B reconstructions and the Alfven resistivity, edge by edge
     ptrdiff_t base1, st1, base2, st2, mv1, mv2, ostr;
@@ -236,32 +252,41 @@ __global__ void __launch_bounds__(kBlock) k_emf_march(Dims d, Coef<R> c, EmfIn<T
     }
     ptrdiff_t eo = K == 1 ? ix(d.nx0, d.ny0 + 1, i, j0, k0)
                           : (K == 2 ? ix(d.nx0 + 1, d.ny0, i, j0, k0) : ix(d.nx0 + 1, d.ny0 + 1, i, j0, k0));
-    ptrdiff_t bc = bc0;
+    // 32-bit byte offsets (every array stays below 2^32 bytes)
+    constexpr unsigned E = sizeof(TI);
+    unsigned ob1 = (unsigned)base1 * E, ob2 = (unsigned)base2 * E, obc = (unsigned)bc0 * E;
+    const unsigned ost1 = (unsigned)st1 * E, ost2 = (unsigned)st2 * E, omv1 = (unsigned)mv1 * E,
+                   omv2 = (unsigned)mv2 * E, omarch = (unsigned)smarch * E;
+    const unsigned osx = E, osy = (unsigned)sy * E, osz = (unsigned)sz * E;
 #pragma unroll
     for (int s = 0; s < T; ++s) {
         if (s < cnt) {
             const R sig1 = sgn1(v1av[s]), sig2 = sgn1(v2av[s]);
             R b1l, b1r, b2l, b2r;
-            recon_line<R, M>(in.b1, base1, st1, c, b1l, b1r);
-            recon_line<R, M>(in.b2, base2, st2, c, b2l, b2r);
+            // b1 runs along x for K=2, b2 for K=3
+            recon_off<R, M, K == 2>(in.b1, ob1, ost1, c, b1l, b1r);
+            recon_off<R, M, K == 3>(in.b2, ob2, ost2, c, b2l, b2r);
             const R b1u = (R(1) + sig2) / R(2) * b1l + (R(1) - sig2) / R(2) * b1r;
             const R b2u = (R(1) + sig1) / R(2) * b2l + (R(1) - sig1) / R(2) * b2r;
             R dum = -v1av[s] * b2u + v2av[s] * b1u;
             R eta;
             if (K == 1)
-                eta = R(.25) * (R(in.vf[bc - sy - sz]) + R(in.vf[bc - sz]) + R(in.vf[bc]) + R(in.vf[bc - sy]));
+                eta = R(.25) * (R(load_at(in.vf, obc - osy - osz)) + R(load_at(in.vf, obc - osz)) +
+                                R(load_at(in.vf, obc)) + R(load_at(in.vf, obc - osy)));
             else if (K == 2)
-                eta = R(.25) * (R(in.vf[bc - sx - sz]) + R(in.vf[bc - sx]) + R(in.vf[bc]) + R(in.vf[bc - sz]));
+                eta = R(.25) * (R(load_at(in.vf, obc - osx - osz)) + R(load_at(in.vf, obc - osx)) +
+                                R(load_at(in.vf, obc)) + R(load_at(in.vf, obc - osz)));
             else
-                eta = R(.25) * (R(in.vf[bc - sx - sy]) + R(in.vf[bc - sy]) + R(in.vf[bc]) + R(in.vf[bc - sx]));
+                eta = R(.25) * (R(load_at(in.vf, obc - osx - osy)) + R(load_at(in.vf, obc - osy)) +
+                                R(load_at(in.vf, obc)) + R(load_at(in.vf, obc - osx)));
             dum = dum + eta * (b1l + b2r - b1r - b2l);
             // ef = |C_a||C_b|/det A of an affine metric (E . a_k per unit
             // computational length), exactly 1 on the reference's grid; a
             // general metric gives E . dl per edge (efa)
             e[eo] = TF(dum * (efa ? R(efa[eo]) : ef));
-            base1 += mv1;
-            base2 += mv2;
-            bc += smarch;
+            ob1 += omv1;
+            ob2 += omv2;
+            obc += omarch;
             eo += ostr;
         }
     }

@@ -1006,6 +1006,14 @@ void* mhd_device_pointer(mhd_ctx* c, int which) {
 int mhd_profile(mhd_ctx* c, int enable) {
     if (!c) return MHD_ERR_PARAM;
     c->prof = enable != 0;
+    c->prof_mask = ~0u;
+    return MHD_OK;
+}
+
+int mhd_profile_classes(mhd_ctx* c, unsigned mask) {
+    if (!c) return MHD_ERR_PARAM;
+    c->prof = mask != 0;
+    c->prof_mask = mask;
     return MHD_OK;
 }
 

@@ -83,6 +83,7 @@ struct mhd_ctx {
     hipError_t launch_error = hipSuccess;
     // kernel timing
     bool prof = false;
+    unsigned prof_mask = ~0u;   // classes timed while prof is set
     struct Rec {
         int kclass;
         hipEvent_t a, b;
@@ -124,7 +125,8 @@ template <typename... KArgs, typename... Args>
 inline void launch(mhd_ctx* c, int kclass, dim3 g, dim3 b, void (*k)(KArgs...), Args... args) {
     if (g.x == 0 || g.y == 0) return;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c->prof) {
+    const bool timed = c->prof && ((c->prof_mask >> kclass) & 1u);
+    if (timed) {
         e0 = ev_get(c);
         e1 = ev_get(c);
         hipEventRecord(e0, c->s);
@@ -132,7 +134,7 @@ inline void launch(mhd_ctx* c, int kclass, dim3 g, dim3 b, void (*k)(KArgs...), 
     hipLaunchKernelGGL(k, g, b, 0, c->s, static_cast<KArgs>(args)...);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess && c->launch_error == hipSuccess) c->launch_error = err;
-    if (c->prof) {
+    if (timed) {
         hipEventRecord(e1, c->s);
         c->evs.push_back({kclass, e0, e1});
     }

@@ -35,6 +35,12 @@ template <>
 __device__ __forceinline__ float qdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 template <typename R>
 __device__ __forceinline__ R qsqrt(R x) { return sqrt(x); }
+// division by a constant: binary64 divides; fp32 multiplies by the rounded
+// reciprocal (folded at compile time)
+template <typename R>
+__device__ __forceinline__ R cdiv(R x, double c) { return x / R(c); }
+template <>
+__device__ __forceinline__ float cdiv<float>(float x, double c) { return x * float(1. / c); }
 template <>
 __device__ __forceinline__ float qsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
@@ -50,9 +56,9 @@ __device__ __forceinline__ R dmin(R a, R b) { return a < b ? a : b; }
 // interp_pdm 8-point face value, s[0..7] = cells -4..+3 (mhd3d3.f90:871-874)
 template <typename R>
 __device__ __forceinline__ R interp8(const R* s) {
-    return (R(-3) * s[0] + R(29) * s[1] - R(139) * s[2] + R(533) * s[3] + R(533) * s[4] - R(139) * s[5] +
-            R(29) * s[6] - R(3) * s[7]) /
-           R(840);
+    return cdiv<R>(R(-3) * s[0] + R(29) * s[1] - R(139) * s[2] + R(533) * s[3] + R(533) * s[4] - R(139) * s[5] +
+                       R(29) * s[6] - R(3) * s[7],
+                   840.);
 }
 
 // PDM2 limiter (mhd3d3.f90:1142-1166)
@@ -73,28 +79,28 @@ __device__ __forceinline__ void pdm2(R fm2, R fm1, R f0, R fp1, R fi, R pdmb, R&
 // TVD van Leer (mhd3d3.f90:1240-1246)
 template <typename R>
 __device__ __forceinline__ void tvd(R fm2, R fm1, R f0, R fp1, R fi, R eps, R& fl, R& fr) {
-    R a = (fm1 - fm2) / (f0 - fm1 + eps);
-    R b = (a + fabs(a)) / (R(1) + fabs(a));
+    R a = qdiv(fm1 - fm2, f0 - fm1 + eps);
+    R b = qdiv(a + fabs(a), R(1) + fabs(a));
     fl = fm1 - b * (fm1 - fi);
-    a = (f0 - fp1) / (fm1 - f0 + eps);
-    b = (a + fabs(a)) / (R(1) + fabs(a));
+    a = qdiv(f0 - fp1, fm1 - f0 + eps);
+    b = qdiv(a + fabs(a), R(1) + fabs(a));
     fr = f0 - b * (f0 - fi);
 }
 
 // one side of WENO5 (mhd3d3.f90:1335-1356); mirrored arguments give the right side
 template <typename R>
 __device__ __forceinline__ R weno5_side(R fm3, R fm2, R fm1, R f0, R fp1, R eps) {
-    R h0 = (R(2) * fm3 - R(7) * fm2 + R(11) * fm1) / R(6);
-    R h1 = (-fm2 + R(5) * fm1 + R(2) * f0) / R(6);
-    R h2 = (R(2) * fm1 + R(5) * f0 - fp1) / R(6);
+    R h0 = cdiv<R>(R(2) * fm3 - R(7) * fm2 + R(11) * fm1, 6.);
+    R h1 = cdiv<R>(-fm2 + R(5) * fm1 + R(2) * f0, 6.);
+    R h2 = cdiv<R>(R(2) * fm1 + R(5) * f0 - fp1, 6.);
     R s0 = R(13. / 12.) * sq(fm3 - R(2) * fm2 + fm1) + R(.25) * sq(fm3 - R(4) * fm2 + R(3) * fm1);
     R s1 = R(13. / 12.) * sq(fm2 - R(2) * fm1 + f0) + R(.25) * sq(fm2 - f0);
     R s2 = R(13. / 12.) * sq(fm1 - R(2) * f0 + fp1) + R(.25) * sq(R(3) * fm1 - R(4) * f0 + fp1);
-    s0 = R(.1) / sq(eps + s0);
-    s1 = R(.6) / sq(eps + s1);
-    s2 = R(.3) / sq(eps + s2);
+    s0 = qdiv(R(.1), sq(eps + s0));
+    s1 = qdiv(R(.6), sq(eps + s1));
+    s2 = qdiv(R(.3), sq(eps + s2));
     R t = s0 + s1 + s2;
-    R w0 = s0 / t, w1 = s1 / t, w2 = s2 / t;
+    R w0 = qdiv(s0, t), w1 = qdiv(s1, t), w2 = qdiv(s2, t);
     return w0 * h0 + w1 * h1 + w2 * h2;
 }
 
@@ -124,7 +130,7 @@ __device__ __forceinline__ R mp_side(R u, R fm3, R fm2, R fm1, R f0, R fp1, R al
     w0 = minmod(w1, w2);
     s1 = fm1 + alpha * (fm1 - fm2);
     s2 = R(.5) * (fm1 + f0 - s0);
-    s0 = R(.5) * (R(3) * fm1 - fm2) + beta * w0 / R(3);
+    s0 = R(.5) * (R(3) * fm1 - fm2) + cdiv<R>(beta * w0, 3.);
     w1 = dmin(dmin(fm1, f0), s2);
     w2 = dmin(dmin(fm1, s1), s0);
     w0 = dmax(w1, w2);
@@ -214,6 +220,25 @@ __device__ __forceinline__ void recon_taps(const T* __restrict__ f, const Taps<R
     R s[S::n];
 #pragma unroll
     for (int m = 0; m < S::n; ++m) s[m] = R(load_at(f, tp.o[m]));
+    S::run(s, c, fl, fr);
+}
+
+// recon_line from a byte offset and a byte stride (32-bit); UNIT: the
+// stride is one element, the taps are contiguous and load as wide vectors
+// from one address
+template <typename R, int M, bool UNIT, typename T>
+__device__ __forceinline__ void recon_off(const T* __restrict__ f, unsigned ob, unsigned ost, const Coef<R>& c,
+                                          R& fl, R& fr) {
+    using S = Stencil<R, M>;
+    R s[S::n];
+    if constexpr (UNIT) {
+        const T* p = reinterpret_cast<const T*>(reinterpret_cast<const char*>(f) + ob);
+#pragma unroll
+        for (int m = 0; m < S::n; ++m) s[m] = R(p[S::lo + m]);
+    } else {
+#pragma unroll
+        for (int m = 0; m < S::n; ++m) s[m] = R(load_at(f, ob + (unsigned)(S::lo + m) * ost));
+    }
     S::run(s, c, fl, fr);
 }
 

@@ -42,7 +42,7 @@ void flux_launch(mhd_ctx* c, Src<TI> s) {
 // sliding stencil windows; a launch whose long-run grid has fewer threads
 // than c->emf_fill takes runs of 2 instead, trading window reuse for latency
 // hiding
-template <int M>
+template <int M, int K>
 struct EmfSteps {
     static constexpr int T = M == TVD ? 16 : 8;
     static constexpr int T_SHORT = 2;
@@ -61,13 +61,13 @@ void emf_launch_kt(mhd_ctx* c, EmfIn<TI> in) {
 
 template <typename R, int M, int K, typename TI, typename TF>
 void emf_launch_k(mhd_ctx* c, EmfIn<TI> in) {
-    constexpr int T = EmfSteps<M>::T;
+    constexpr int T = EmfSteps<M, K>::T;
     const Dims& d = c->d;
     const long ii = d.nx0 + (K != 1);
     const long no = K == 3 ? d.nz0 : (K == 1 ? d.ny0 + 1 : d.ny0);
     const long nm = K == 3 ? d.ny0 + 1 : d.nz0 + 1;
     if (ii * no * ((nm + T - 1) / T) >= c->emf_fill) emf_launch_kt<R, M, K, T, TI, TF>(c, in);
-    else emf_launch_kt<R, M, K, EmfSteps<M>::T_SHORT, TI, TF>(c, in);
+    else emf_launch_kt<R, M, K, EmfSteps<M, K>::T_SHORT, TI, TF>(c, in);
 }
 
 // edge k: (v1, v2, b1, b2) = (vy, vz, bj, bk), (vz, vx, bk, bi), (vx, vy, bi, bj);

@@ -67,6 +67,7 @@ SIGNATURES = {
     "mhd_device_pointer": (C.c_void_p, [C.c_void_p, C.c_int]),
     "mhd_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "mhd_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]),
+    "mhd_profile_classes": (C.c_int, [C.c_void_p, C.c_uint]),
     "mhd_surrogate_set_weights": (C.c_int, [C.c_void_p, _F, C.c_int]),
     "mhd_surrogate_get_weights": (C.c_int, [C.c_void_p, _F, C.c_int]),
     "mhd_surrogate_configure": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
@@ -263,6 +264,13 @@ class Solver:
 
     def profile(self, on=True):
         _ck(lib().mhd_profile(self.h, 1 if on else 0), "mhd_profile")
+
+    def profile_classes(self, classes):
+        """time only the given kernel classes (names or MHD_K_* codes)"""
+        m = 0
+        for k in classes:
+            m |= 1 << (KCLASS[k] if isinstance(k, str) else k)
+        _ck(lib().mhd_profile_classes(self.h, m), "mhd_profile_classes")
 
     def profile_read(self, kclass):
         n = C.c_int(0)
