@@ -222,6 +222,73 @@ def shell_config(steps, warmup=2):
             "initial_max_divB_normalised": float(np.abs(shell.div_b(g, st)).max()) * hmin / bmax}
 
 
+ENSEMBLE_SIZES = (32, 48, 64, 96)
+
+
+def ensemble_config(steps, nbox=16, warmup=2):
+    """batched ensemble: nbox independent periodic boxes with edges drawn
+    (SplitMix64, seed 2) from 32, 48, 64, 96; Gaussian blobs (rho = 1 + 3
+    seeded blobs, p = 1, uniform |B| = 0.5, v = 0), TVD, RK2, fixed dt =
+    0.5 x each box's initial CFL-0.3 dt.  The boxes step together, one HIP
+    graph launch per step (mhd_ensemble_*), in fp32 stencils + binary64
+    accumulation and in binary64; the mixed ensemble is also timed box by
+    box (mhd_step on each box in turn) for comparison."""
+    u = ics.uniform01(2, nbox)
+    sizes = [ENSEMBLE_SIZES[min(int(x * 4), 3)] for x in u]
+    zones = sum(n ** 3 for n in sizes)
+    rec = {"workload": "%d periodic boxes, edges %s; Gaussian blobs, TVD, RK2, %d steps (+%d warm-up), dt = 0.5 x "
+                       "initial CFL dt per box" % (nbox, sizes, steps, warmup, ), "zones": zones}
+    states = {}
+    for prec in ("mixed", "fp64"):
+        solvers, taus, grids = [], [], []
+        for b, n in enumerate(sizes):
+            g = ics.Grid(n, n, n, ng=3)
+            st = ics.initial_state(g, "blobs", 100 + b)
+            s = gpu.Solver(gpu.make_params(n, n, n, ng=3, reconop=2, stepop=2, bc="periodic", precision=prec), g)
+            s.upload(st)
+            s.initialize(0.)
+            taus.append(.5 * s.get_dt())
+            solvers.append(s)
+            grids.append(g)
+        try:
+            with gpu.Ensemble(solvers, taus) as ens:
+                ens.step(warmup)
+                t0 = time.perf_counter()
+                ens.step(steps)
+                wall = time.perf_counter() - t0
+            r = {"ensemble": {"value": zones * steps / wall, "ms_per_step": wall / steps * 1e3}}
+            if prec == "mixed":
+                # the same boxes, one after the other with their own launches
+                for s, tau in zip(solvers, taus):
+                    s.step(warmup, tau)
+                for s in solvers:
+                    s.synchronize()
+                t0 = time.perf_counter()
+                for s, tau in zip(solvers, taus):
+                    s.step(steps, tau)
+                wall = time.perf_counter() - t0
+                r["box_by_box"] = {"value": zones * steps / wall, "ms_per_step": wall / steps * 1e3}
+                r["ensemble_speedup_vs_box_by_box"] = r["ensemble"]["value"] / r["box_by_box"]["value"]
+                # the comparison with binary64 is taken at the ensemble's
+                # step count: re-run the mixed ensemble from the start
+                for s, g, b in zip(solvers, grids, range(nbox)):
+                    s.upload(ics.initial_state(g, "blobs", 100 + b))
+                    s.initialize(0.)
+                with gpu.Ensemble(solvers, taus) as ens:
+                    ens.step(warmup + steps)
+            states[prec] = [(g, s.download()) for g, s in zip(grids, solvers)]
+            rec[prec] = r
+        finally:
+            for s in solvers:
+                s.close()
+    num = sum(float(np.abs(g.interior(a["rho"]) - g.interior(b["rho"])).sum())
+              for (g, a), (_, b) in zip(states["mixed"], states["fp64"]))
+    den = sum(float(np.abs(g.interior(b["rho"])).sum()) for g, b in states["fp64"])
+    rec["density_L1_mixed_vs_fp64"] = num / den
+    rec["speedup_mixed_vs_fp64"] = rec["mixed"]["ensemble"]["value"] / rec["fp64"]["ensemble"]["value"]
+    return rec
+
+
 def host_cores():
     try:
         n = len(os.sched_getaffinity(0))
@@ -302,6 +369,9 @@ def main():
     ap.add_argument("--shell-steps", type=int, default=200,
                     help="steps of the spherical-shell side record (0: skip it)")
     ap.add_argument("--shell-only", action="store_true", help="print only the spherical-shell record")
+    ap.add_argument("--ensemble-steps", type=int, default=100,
+                    help="steps of the batched-ensemble side record (0: skip it)")
+    ap.add_argument("--ensemble-only", action="store_true", help="print only the batched-ensemble record")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the MPI decomposition of the reference is replaced by a single "
@@ -309,6 +379,9 @@ def main():
         sys.exit(2)
     if args.shell_only:
         print(json.dumps(shell_config(args.shell_steps)), flush=True)
+        return
+    if args.ensemble_only:
+        print(json.dumps(ensemble_config(args.ensemble_steps)), flush=True)
         return
     n = args.n
     if args.theta is not None:
@@ -405,6 +478,9 @@ def main():
     if args.shell_steps > 0:
         log("spherical-shell record (%d steps) ..." % args.shell_steps)
         out["shell_config"] = shell_config(args.shell_steps)
+    if args.ensemble_steps > 0:
+        log("batched-ensemble record (%d steps) ..." % args.ensemble_steps)
+        out["ensemble_config"] = ensemble_config(args.ensemble_steps)
     if not args.no_cpu_baseline:
         cores = host_cores()
         ranks = 16 if cores >= 16 else 8

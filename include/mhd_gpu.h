@@ -125,6 +125,22 @@ int mhd_initialize(mhd_ctx *ctx, double *tau);
  * reference warns when tau_cfl < tau0 for RK, mhd3d3.f90:3658-3661). */
 int mhd_step(mhd_ctx *ctx, int nsteps, double *tau, double *tau_cfl);
 
+/* batched ensemble of independent blocks (no counterpart in the reference,
+ * which runs one domain per MPI job): n contexts of any sizes and
+ * precisions on one device, stepped together by one HIP graph launch per
+ * step whose per-member branches run concurrently.  Members must use a
+ * Runge-Kutta step (stepop 2 or 3: in place, fixed tau), no learned
+ * reconstruction, and be initialised (mhd_initialize); tau[m] is member m's
+ * fixed step, as *tau of mhd_step.  The graph is recorded once, from the
+ * members' launch sequences at creation.  Members stay usable on their
+ * own between ensemble steps (upload, download, mhd_step); a member must
+ * not be destroyed before the ensemble.  MHD_ERR_PARAM / MHD_ERR_UNSUPPORTED
+ * for an empty or mixed-device set or an unsupported member. */
+typedef struct mhd_ens mhd_ens;
+int mhd_ensemble_create(mhd_ens **ens, mhd_ctx *const *members, int n, const double *tau);
+int mhd_ensemble_step(mhd_ens *ens, int nsteps);
+void mhd_ensemble_destroy(mhd_ens *ens);
+
 /* per-routine drop-ins, each acting on the context's current state */
 int mhd_get_dt(mhd_ctx *ctx, double *tau_cfl);   /* get_dt, mhd3d3.f90:1608 */
 int mhd_fill_ghosts(mhd_ctx *ctx);                /* fill_hydro_mpi + fill_mag_mpi, 781-819 */

@@ -898,6 +898,102 @@ int mhd_step(mhd_ctx* c, int nsteps, double* tau, double* tau_cfl) {
     return rc;
 }
 
+// ---------------------------------------------------------------------------
+// batched ensemble: one graph whose branches are the members' RK steps
+// ---------------------------------------------------------------------------
+struct mhd_ens {
+    std::vector<mhd_ctx*> m;
+    int device = 0;
+    hipStream_t root = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+void mhd_ensemble_destroy(mhd_ens* e) {
+    if (!e) return;
+    hipSetDevice(e->device);
+    if (e->root) hipStreamSynchronize(e->root);
+    if (e->exec) hipGraphExecDestroy(e->exec);
+    if (e->graph) hipGraphDestroy(e->graph);
+    for (hipEvent_t j : e->join)
+        if (j) hipEventDestroy(j);
+    if (e->fork) hipEventDestroy(e->fork);
+    if (e->root) hipStreamDestroy(e->root);
+    delete e;
+}
+
+int mhd_ensemble_create(mhd_ens** out, mhd_ctx* const* members, int n, const double* tau) {
+    if (!out || !members || !tau || n < 1) return MHD_ERR_PARAM;
+    *out = nullptr;
+    const int dev = members[0] ? members[0]->p.device : 0;
+    for (int a = 0; a < n; ++a) {
+        mhd_ctx* c = members[a];
+        if (!c || !c->have_geometry || c->p.device != dev) return MHD_ERR_PARAM;
+        for (int b = 0; b < a; ++b)
+            if (members[b] == c) return MHD_ERR_PARAM;
+        if (c->p.stepop == 1 || c->p.surrogate) return MHD_ERR_UNSUPPORTED;
+    }
+    if (hipSetDevice(dev) != hipSuccess) return MHD_ERR_HIP;
+    mhd_ens* e = new mhd_ens();
+    e->m.assign(members, members + n);
+    e->device = dev;
+    e->join.assign(n, nullptr);
+    bool ok = hipStreamCreateWithFlags(&e->root, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) == hipSuccess;
+    for (int a = 0; ok && a < n; ++a) ok = hipEventCreateWithFlags(&e->join[a], hipEventDisableTiming) == hipSuccess;
+    // finish whatever the members have queued, then record one step of each
+    // member on its own stream, forked from and joined into the root stream
+    for (int a = 0; ok && a < n; ++a) ok = hipStreamSynchronize(e->m[a]->s) == hipSuccess;
+    if (!ok) {
+        mhd_ensemble_destroy(e);
+        return MHD_ERR_HIP;
+    }
+    std::vector<bool> prof(n);
+    for (int a = 0; a < n; ++a) {
+        prof[a] = e->m[a]->prof;
+        e->m[a]->prof = false;   // no event records inside the graph
+        e->m[a]->tau0_host = tau[a];
+    }
+    ok = hipStreamBeginCapture(e->root, hipStreamCaptureModeThreadLocal) == hipSuccess &&
+         hipEventRecord(e->fork, e->root) == hipSuccess;
+    for (int a = 0; ok && a < n; ++a) {
+        mhd_ctx* c = e->m[a];
+        ok = hipStreamWaitEvent(c->s, e->fork, 0) == hipSuccess;
+        if (!ok) break;
+        if (c->p.stepop == 2) step_rk<2>(c);
+        else step_rk<3>(c);
+        ok = c->launch_error == hipSuccess && hipEventRecord(e->join[a], c->s) == hipSuccess &&
+             hipStreamWaitEvent(e->root, e->join[a], 0) == hipSuccess;
+    }
+    hipGraph_t g = nullptr;
+    const bool ended = hipStreamEndCapture(e->root, &g) == hipSuccess;
+    e->graph = g;
+    for (int a = 0; a < n; ++a) {
+        e->m[a]->prof = prof[a];
+        e->m[a]->launch_error = hipSuccess;
+    }
+    ok = ok && ended && g && hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0) == hipSuccess;
+    if (!ok) {
+        mhd_ensemble_destroy(e);
+        return MHD_ERR_HIP;
+    }
+    *out = e;
+    return MHD_OK;
+}
+
+int mhd_ensemble_step(mhd_ens* e, int nsteps) {
+    if (!e || nsteps < 0) return MHD_ERR_PARAM;
+    hipSetDevice(e->device);
+    // members' own queued work first, then the graph on the root stream
+    for (mhd_ctx* c : e->m)
+        if (hipStreamSynchronize(c->s) != hipSuccess) return MHD_ERR_HIP;
+    for (int s = 0; s < nsteps; ++s)
+        if (hipGraphLaunch(e->exec, e->root) != hipSuccess) return MHD_ERR_HIP;
+    return hipStreamSynchronize(e->root) == hipSuccess ? MHD_OK : MHD_ERR_HIP;
+}
+
 int mhd_fetch(mhd_ctx* c, int which, double* out) {
     if (!c || !out) return MHD_ERR_PARAM;
     hipSetDevice(c->p.device);

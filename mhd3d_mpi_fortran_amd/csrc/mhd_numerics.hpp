@@ -79,12 +79,27 @@ __device__ __forceinline__ void pdm2(R fm2, R fm1, R f0, R fp1, R fi, R pdmb, R&
 // TVD van Leer (mhd3d3.f90:1240-1246)
 template <typename R>
 __device__ __forceinline__ void tvd(R fm2, R fm1, R f0, R fp1, R fi, R eps, R& fl, R& fr) {
-    R a = qdiv(fm1 - fm2, f0 - fm1 + eps);
-    R b = qdiv(a + fabs(a), R(1) + fabs(a));
+    R a = (fm1 - fm2) / (f0 - fm1 + eps);
+    R b = (a + fabs(a)) / (R(1) + fabs(a));
     fl = fm1 - b * (fm1 - fi);
-    a = qdiv(f0 - fp1, fm1 - f0 + eps);
-    b = qdiv(a + fabs(a), R(1) + fabs(a));
+    a = (f0 - fp1) / (fm1 - f0 + eps);
+    b = (a + fabs(a)) / (R(1) + fabs(a));
     fr = f0 - b * (f0 - fi);
+}
+
+// fp32 (mixed precision): the same limiter b(a) = (a+|a|)/(1+|a|) with
+// a = n/d written as b = 2n/(n+d) for a > 0 and 0 otherwise.  In fp32 the
+// denominator d = f(0) - f(-1) + eps can cancel exactly (eps = 1e-8 is within
+// the rounding of velocity differences), where the reference's form gives
+// inf/inf; here a = +inf gives its limit b = 2 and 0/0 gives b = 0.
+__device__ __forceinline__ float tvd_b(float n, float d) {
+    return (n != 0.f && ((n > 0.f) == (d >= 0.f))) ? 2.f * n * __builtin_amdgcn_rcpf(n + d) : 0.f;
+}
+template <>
+__device__ __forceinline__ void tvd<float>(float fm2, float fm1, float f0, float fp1, float fi, float eps, float& fl,
+                                           float& fr) {
+    fl = fm1 - tvd_b(fm1 - fm2, f0 - fm1 + eps) * (fm1 - fi);
+    fr = f0 - tvd_b(f0 - fp1, fm1 - f0 + eps) * (f0 - fi);
 }
 
 // one side of WENO5 (mhd3d3.f90:1335-1356); mirrored arguments give the right side

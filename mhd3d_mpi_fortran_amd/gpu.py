@@ -68,6 +68,9 @@ SIGNATURES = {
     "mhd_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "mhd_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]),
     "mhd_profile_classes": (C.c_int, [C.c_void_p, C.c_uint]),
+    "mhd_ensemble_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int, _D]),
+    "mhd_ensemble_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "mhd_ensemble_destroy": (None, [C.c_void_p]),
     "mhd_surrogate_set_weights": (C.c_int, [C.c_void_p, _F, C.c_int]),
     "mhd_surrogate_get_weights": (C.c_int, [C.c_void_p, _F, C.c_int]),
     "mhd_surrogate_configure": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
@@ -278,6 +281,39 @@ class Solver:
         _ck(lib().mhd_profile_read(self.h, KCLASS[kclass] if isinstance(kclass, str) else kclass, C.byref(n),
                                    C.byref(ms)), "mhd_profile_read")
         return n.value, ms.value
+
+
+class Ensemble:
+    """independent Runge-Kutta solvers of any sizes and precisions stepped
+    together, one HIP graph launch per step (mhd_ensemble_*); taus[m] is
+    solver m's fixed step"""
+
+    def __init__(self, solvers, taus):
+        self.solvers = list(solvers)
+        arr = (C.c_void_p * len(self.solvers))(*[s.h for s in self.solvers])
+        t = np.ascontiguousarray(taus, dtype=np.float64)
+        if t.shape != (len(self.solvers),):
+            raise ValueError("one tau per solver")
+        h = C.c_void_p()
+        _ck(lib().mhd_ensemble_create(C.byref(h), arr, len(self.solvers), _p(t)), "mhd_ensemble_create")
+        self.h = h
+
+    def step(self, nsteps=1):
+        _ck(lib().mhd_ensemble_step(self.h, int(nsteps)), "mhd_ensemble_step")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mhd_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def reconstruct(solver, f, lshape, direction):
