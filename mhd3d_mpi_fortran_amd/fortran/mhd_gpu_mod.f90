@@ -115,6 +115,31 @@ module mhd_gpu
          type(c_ptr), value :: ctx
          real(c_double), intent(in) :: a(9)
       end function
+      ! general curvilinear metric from the (nx+1)(ny+1)(nz+1) node
+      ! coordinates, column-major, ghosts included (no reference counterpart)
+      integer(c_int) function mhd_set_metric_nodes(ctx, x, y, z) bind(C, name='mhd_set_metric_nodes')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: x(*), y(*), z(*)
+      end function
+      ! batched ensemble of Runge-Kutta contexts, one graph launch per step
+      ! (no reference counterpart)
+      integer(c_int) function mhd_ensemble_create(ens, members, n, tau) bind(C, name='mhd_ensemble_create')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), intent(out) :: ens
+         type(c_ptr), intent(in) :: members(*)
+         integer(c_int), value :: n
+         real(c_double), intent(in) :: tau(*)
+      end function
+      integer(c_int) function mhd_ensemble_step(ens, nsteps) bind(C, name='mhd_ensemble_step')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ens
+         integer(c_int), value :: nsteps
+      end function
+      subroutine mhd_ensemble_destroy(ens) bind(C, name='mhd_ensemble_destroy')
+         import :: c_ptr
+         type(c_ptr), value :: ens
+      end subroutine
       ! learned reconstruction (no reference counterpart)
       integer(c_int) function mhd_surrogate_set_weights(ctx, w, n) bind(C, name='mhd_surrogate_set_weights')
          import :: c_int, c_ptr, c_float
