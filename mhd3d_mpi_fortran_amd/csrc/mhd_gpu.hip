@@ -161,6 +161,13 @@ int mhd_create(mhd_ctx** out, const mhd_params* p) {
         delete c;
         return MHD_ERR_HIP;
     }
+    if (hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_emf, hipEventDisableTiming) != hipSuccess) {
+        mhd_destroy(c);
+        return MHD_ERR_HIP;
+    }
     // zero everything once so ghost shells and unused slots hold defined values
     if (hipMemsetAsync(c->arena, 0, total, c->s) != hipSuccess || hipStreamSynchronize(c->s) != hipSuccess) {
         mhd_destroy(c);
@@ -174,6 +181,7 @@ void mhd_destroy(mhd_ctx* c) {
     if (!c) return;
     hipSetDevice(c->p.device);
     if (c->s) hipStreamSynchronize(c->s);
+    if (c->s2) hipStreamSynchronize(c->s2);
     for (auto& e : c->evpool) hipEventDestroy(e);
     for (auto& r : c->evs) {
         hipEventDestroy(r.a);
@@ -183,6 +191,10 @@ void mhd_destroy(mhd_ctx* c) {
     if (c->metbuf) hipFree(c->metbuf);
     if (c->gmetbuf) hipFree(c->gmetbuf);
     if (c->arena) hipFree(c->arena);
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->ev_emf) hipEventDestroy(c->ev_emf);
+    if (c->s2) hipStreamDestroy(c->s2);
     if (c->s) hipStreamDestroy(c->s);
     delete c;
 }
@@ -671,10 +683,10 @@ static void do_prev_rk(mhd_ctx* c) {
            c->bjp, c->bkp, c->ncell, c->nbi, c->nbj, c->nbk);
 }
 
-// conserved update + CT from the current state into `out` (the current
-// state itself for RK, the next-state buffers for AB)
+// conserved update from the current state into `out` (the current state
+// itself for RK, the next-state buffers for AB)
 template <typename TF, int NST, int ST>
-static void do_update_t(mhd_ctx* c, double* const* out) {
+static void do_cells_t(mhd_ctx* c, double* const* out) {
     CCells s{c->rho, c->vx, c->vy, c->vz, c->pr};
     Cells so{out[0], out[1], out[2], out[3], out[4]};
     PrevRK pv{c->rhop, c->mxp, c->myp, c->mzp, c->engp};
@@ -683,32 +695,76 @@ static void do_update_t(mhd_ctx* c, double* const* out) {
     auto ku = gen ? k_update<NST, ST, TF, true> : k_update<NST, ST, TF, false>;
     launch(c, MHD_K_UPDATE, blocks(c->ninterior), dim3(kBlock), ku, c->d, c->p.gam - 1.,
            (const double*)(c->scal + SC_TAU), s, so, pv, F, sdx(c), sdy(c), sdz(c), gmet(c), c->ninterior);
+}
+
+// CT face update from the stage's EMFs into `out`
+template <typename TF, int NST, int ST>
+static void do_ct_t(mhd_ctx* c, double* const* out) {
     CFaces b{c->bi, c->bj, c->bk};
     Faces bo{out[8], out[9], out[10]};
     CFaces bp{c->bip, c->bjp, c->bkp};
     const long nct = (long)(c->d.nx0 + 1) * c->d.ny0 * c->d.nz0 + (long)c->d.nx0 * (c->d.ny0 + 1) * c->d.nz0 +
                      (long)c->d.nx0 * c->d.ny0 * (c->d.nz0 + 1);
     const CtFactor ctf{{c->met.ct[0], c->met.ct[1], c->met.ct[2]}};
-    auto kc = gen ? k_ct<NST, ST, TF, true> : k_ct<NST, ST, TF, false>;
+    auto kc = c->met.mode == 2 ? k_ct<NST, ST, TF, true> : k_ct<NST, ST, TF, false>;
     launch(c, MHD_K_CT, blocks(nct), dim3(kBlock), kc, c->d, (const double*)(c->scal + SC_TAU), ctf, b,
            bo, bp, (const TF*)c->emf[0], (const TF*)c->emf[1], (const TF*)c->emf[2], (const double*)c->dx,
            (const double*)c->dy, (const double*)c->dz, gmet(c));
 }
 
-template <int NST, int ST>
-static void do_update(mhd_ctx* c, double* const* out) {
-    if (mixed(c)) do_update_t<float, NST, ST>(c, out);
-    else do_update_t<double, NST, ST>(c, out);
+// One stage's stencil work and update on two streams.  The edge EMF and,
+// behind it, the CT face update run on the second stream; the face flux and
+// the cell update on the main stream.  The EMF reads the stage's
+// velocities, faces and vf and writes the EMF arrays; CT reads those and the
+// faces and writes the faces of `out`; the flux reads the stage's cell
+// primitives and writes the flux arrays; the update reads the fluxes and
+// cells and writes the cells of `out`.  So the only ordering needed across
+// the streams: the update (which may overwrite the cell velocities the
+// binary64 EMF reads) waits for the EMF, and the main stream waits for CT
+// before the stage ends.  The EMF kernels wait on latency (few waves per
+// SIMD) and CT on HBM, the flux on VALU issue: together they fill the CUs
+// better than one after the other.
+template <int NST, int ST, typename E, typename F>
+static void stage_overlapped(mhd_ctx* c, E emf, F flux, double* const* out) {
+    auto cells = [&] {
+        if (mixed(c)) do_cells_t<float, NST, ST>(c, out);
+        else do_cells_t<double, NST, ST>(c, out);
+    };
+    auto ct = [&] {
+        if (mixed(c)) do_ct_t<float, NST, ST>(c, out);
+        else do_ct_t<double, NST, ST>(c, out);
+    };
+    const bool ok = c->overlap && hipEventRecord(c->ev_fork, c->s) == hipSuccess &&
+                    hipStreamWaitEvent(c->s2, c->ev_fork, 0) == hipSuccess;
+    if (!ok) {   // serial
+        emf();
+        flux();
+        cells();
+        ct();
+        return;
+    }
+    c->cur = c->s2;
+    emf();
+    hipEventRecord(c->ev_emf, c->s2);
+    ct();
+    hipEventRecord(c->ev_join, c->s2);
+    c->cur = nullptr;
+    flux();
+    hipStreamWaitEvent(c->s, c->ev_emf, 0);
+    cells();
+    hipStreamWaitEvent(c->s, c->ev_join, 0);
 }
 
-static void do_ops64(mhd_ctx* c, Src<double> s) {
-    launch_emf_fp64(c, s);
-    launch_flux_fp64(c, s);
-}
-static void do_ops32(mhd_ctx* c) {
-    launch_emf_fp32(c, sh_src(c));
-    if (c->p.surrogate) launch_flux_surrogate(c, sh_src(c));
-    else launch_flux_fp32(c, sh_src(c));
+template <int NST, int ST>
+static void do_stage(mhd_ctx* c, Src<double> s64, double* const* out) {
+    if (mixed(c)) {
+        stage_overlapped<NST, ST>(c, [&] { launch_emf_fp32(c, sh_src(c)); }, [&] {
+            if (c->p.surrogate) launch_flux_surrogate(c, sh_src(c));
+            else launch_flux_fp32(c, sh_src(c));
+        }, out);
+    } else {
+        stage_overlapped<NST, ST>(c, [&] { launch_emf_fp64(c, s64); }, [&] { launch_flux_fp64(c, s64); }, out);
+    }
 }
 
 static int sync_status(mhd_ctx* c) {
@@ -824,11 +880,9 @@ static void step_rk(mhd_ctx* c) {
     double* self[11];
     for (int a = 0; a < 11; ++a) self[a] = *state_slot(c, a);
     for (int st = 1; st <= NST; ++st) {
-        if (mixed(c)) do_ops32(c);
-        else do_ops64(c, cur_src(c));
-        if (st == 1) do_update<NST, 1>(c, self);
-        else if (st == 2) do_update<NST, 2>(c, self);
-        else do_update<NST, 3>(c, self);
+        if (st == 1) do_stage<NST, 1>(c, cur_src(c), self);
+        else if (st == 2) do_stage<NST, 2>(c, cur_src(c), self);
+        else do_stage<NST, 3>(c, cur_src(c), self);
         do_fill(c);
         end_of_stage(c);
     }
@@ -854,7 +908,6 @@ static void step_ab(mhd_ctx* c) {
         launch(c, MHD_K_PREV, blocks(c->nmax), dim3(kBlock), k_predict_ab_mixed, c->d, s, fb, bmet(c, false, false),
                (const double*)(c->scal + SC_TAU), c->sh[8], pj, nc, nbi, nbj, nbk);
         launch(c, MHD_K_PREV, dim3(1), dim3(64), k_tau0_from_tau, c->scal + SC_TAU);
-        do_ops32(c);
     } else {
         AB11<double> s;
         for (int a = 0; a < 11; ++a) {
@@ -866,9 +919,8 @@ static void step_ab(mhd_ctx* c) {
         launch(c, MHD_K_PREV, blocks(c->nmax), dim3(kBlock), k_predict_ab<double>, s,
                (const double*)(c->scal + SC_TAU), c->vfh, pj, nc, nbi, nbj, nbk);
         launch(c, MHD_K_PREV, dim3(1), dim3(64), k_tau0_from_tau, c->scal + SC_TAU);
-        do_ops64(c, h_src(c));
     }
-    do_update<0, 0>(c, c->xn);
+    do_stage<0, 0>(c, h_src(c), c->xn);
     // rotate: previous <- current <- next <- previous
     for (int a = 0; a < 11; ++a) {
         double* cur = *state_slot(c, a);
@@ -1103,6 +1155,12 @@ int mhd_profile(mhd_ctx* c, int enable) {
     if (!c) return MHD_ERR_PARAM;
     c->prof = enable != 0;
     c->prof_mask = ~0u;
+    return MHD_OK;
+}
+
+int mhd_set_stage_overlap(mhd_ctx* c, int on) {
+    if (!c) return MHD_ERR_PARAM;
+    c->overlap = on != 0;
     return MHD_OK;
 }
 

@@ -46,6 +46,12 @@ struct mhd_ctx {
     mhd_params p{};
     mhd::Dims d{};
     hipStream_t s = nullptr;
+    // second stream: a stage's edge EMF and CT run on it beside the face flux
+    // and the cell update (see stage_overlapped).  cur: the stream launch()
+    // targets (null = s)
+    hipStream_t s2 = nullptr, cur = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_emf = nullptr, ev_join = nullptr;
+    bool overlap = true;   // false: one stream, kernels in the reference's order
     char* arena = nullptr;
     size_t arena_bytes = 0;
     bool have_geometry = false;
@@ -126,16 +132,17 @@ inline void launch(mhd_ctx* c, int kclass, dim3 g, dim3 b, void (*k)(KArgs...), 
     if (g.x == 0 || g.y == 0) return;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const bool timed = c->prof && ((c->prof_mask >> kclass) & 1u);
+    hipStream_t st = c->cur ? c->cur : c->s;
     if (timed) {
         e0 = ev_get(c);
         e1 = ev_get(c);
-        hipEventRecord(e0, c->s);
+        hipEventRecord(e0, st);
     }
-    hipLaunchKernelGGL(k, g, b, 0, c->s, static_cast<KArgs>(args)...);
+    hipLaunchKernelGGL(k, g, b, 0, st, static_cast<KArgs>(args)...);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess && c->launch_error == hipSuccess) c->launch_error = err;
     if (timed) {
-        hipEventRecord(e1, c->s);
+        hipEventRecord(e1, st);
         c->evs.push_back({kclass, e0, e1});
     }
 }

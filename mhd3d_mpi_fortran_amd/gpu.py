@@ -68,6 +68,7 @@ SIGNATURES = {
     "mhd_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "mhd_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]),
     "mhd_profile_classes": (C.c_int, [C.c_void_p, C.c_uint]),
+    "mhd_set_stage_overlap": (C.c_int, [C.c_void_p, C.c_int]),
     "mhd_ensemble_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int, _D]),
     "mhd_ensemble_step": (C.c_int, [C.c_void_p, C.c_int]),
     "mhd_ensemble_destroy": (None, [C.c_void_p]),
@@ -267,6 +268,10 @@ class Solver:
 
     def profile(self, on=True):
         _ck(lib().mhd_profile(self.h, 1 if on else 0), "mhd_profile")
+
+    def set_overlap(self, on=True):
+        """EMF and CT on a second stream beside the flux and update (default)"""
+        _ck(lib().mhd_set_stage_overlap(self.h, 1 if on else 0), "mhd_set_stage_overlap")
 
     def profile_classes(self, classes):
         """time only the given kernel classes (names or MHD_K_* codes)"""
