@@ -905,7 +905,9 @@ static void step_ab(mhd_ctx* c) {
         for (int a = 8; a < 11; ++a) s.xh[a] = c->sh[a + 1];
         const Proj<float> pj = proj<float>(c, c->w32);
         const FaceB fb{c->bi, c->bj, c->bk, c->xp[8], c->xp[9], c->xp[10]};
-        launch(c, MHD_K_PREV, blocks(c->nmax), dim3(kBlock), k_predict_ab_mixed, c->d, s, fb, bmet(c, false, false),
+        const Dims& d = c->d;
+        const long npred = nc + (long)d.ny * d.nz + (long)d.nx * d.nz + (long)d.nx * d.ny;
+        launch(c, MHD_K_PREV, blocks(npred), dim3(kBlock), k_predict_ab_mixed, c->d, s, fb, bmet(c, false, false),
                (const double*)(c->scal + SC_TAU), c->sh[8], pj, nc, nbi, nbj, nbk);
         launch(c, MHD_K_PREV, dim3(1), dim3(64), k_tau0_from_tau, c->scal + SC_TAU);
     } else {

@@ -588,12 +588,18 @@ struct FaceB {
     const double *bi, *bj, *bk, *bip, *bjp, *bkp;
 };
 
+// threads: one per cell (ghost-inclusive), then one per face that is no
+// cell's lower face (i = nx+1 of bi, j = ny+1 of bj, k = nz+1 of bk).  A
+// cell's thread extrapolates its lower faces once and uses them both for
+// the face copies and for its cell B, so every face is read once, plus the
+// upper neighbours of cell B (row and plane neighbours, mostly from L2).
 __global__ void __launch_bounds__(kBlock) k_predict_ab_mixed(Dims d, AB11<float> s, FaceB fb, BMet met,
                                                             const double* __restrict__ tau_p,
                                                             float* __restrict__ vfh, Proj<float> pj, long ncell,
                                                             long nbi, long nbj, long nbk) {
-    const long t = gtid();
+    const long t = gtid_xcd();
     const double c = tau_p[0] / tau_p[1] / 2.;
+    auto ext = [&](const double* x, const double* xp, ptrdiff_t f) { return x[f] + c * (x[f] - xp[f]); };
     if (t < ncell) {
         double h[8];
 #pragma unroll
@@ -604,15 +610,18 @@ __global__ void __launch_bounds__(kBlock) k_predict_ab_mixed(Dims d, AB11<float>
         }
         int i, j, k;
         unflat(t, d.nx, d.ny, i, j, k);
-        auto ext = [&](const double* x, const double* xp, ptrdiff_t f) { return x[f] + c * (x[f] - xp[f]); };
         const ptrdiff_t fi = ix(d.nx + 1, d.ny, i, j, k), fj = ix(d.nx, d.ny + 1, i, j, k);
-        const double ax = .5 * (ext(fb.bi, fb.bip, fi + 1) + ext(fb.bi, fb.bip, fi));
-        const double ay = .5 * (ext(fb.bj, fb.bjp, fj + d.nx) + ext(fb.bj, fb.bjp, fj));
-        const double az = .5 * (ext(fb.bk, fb.bkp, t + (ptrdiff_t)d.nx * d.ny) + ext(fb.bk, fb.bkp, t));
+        const double bi0 = ext(fb.bi, fb.bip, fi), bi1 = ext(fb.bi, fb.bip, fi + 1);
+        const double bj0 = ext(fb.bj, fb.bjp, fj), bj1 = ext(fb.bj, fb.bjp, fj + d.nx);
+        const double bk0 = ext(fb.bk, fb.bkp, t), bk1 = ext(fb.bk, fb.bkp, t + (ptrdiff_t)d.nx * d.ny);
+        s.xh[8][fi] = float(bi0);
+        s.xh[9][fj] = float(bj0);
+        s.xh[10][t] = float(bk0);
+        const double ax = .5 * (bi1 + bi0);
+        const double ay = .5 * (bj1 + bj0);
+        const double az = .5 * (bk1 + bk0);
         if (met.g.ivol) {
-            const double beta[6] = {ext(fb.bi, fb.bip, fi), ext(fb.bi, fb.bip, fi + 1), ext(fb.bj, fb.bjp, fj),
-                                    ext(fb.bj, fb.bjp, fj + d.nx), ext(fb.bk, fb.bkp, t),
-                                    ext(fb.bk, fb.bkp, t + (ptrdiff_t)d.nx * d.ny)};
+            const double beta[6] = {bi0, bi1, bj0, bj1, bk0, bk1};
             gen_cellb(met.g, t, beta, h + 5);
         } else if (met.on) {
             h[5] = met.M[0] * ax + met.M[1] * ay + met.M[2] * az;
@@ -627,14 +636,27 @@ __global__ void __launch_bounds__(kBlock) k_predict_ab_mixed(Dims d, AB11<float>
         for (int a = 5; a < 8; ++a) s.xh[a][t] = float(h[a]);
         vfh[t] = float(sqrt(h[1] * h[1] + h[2] * h[2] + h[3] * h[3]) + sqrt((h[5] * h[5] + h[6] * h[6] + h[7] * h[7]) / h[0]));
         if (pj.on) project(pj, t, h[1], h[2], h[3]);
+        return;
     }
-    const long nf[3] = {nbi, nbj, nbk};
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-        if (t < nf[a]) {
-            const double x = s.x[8 + a][t], xp = s.xp[8 + a][t];
-            s.xh[8 + a][t] = float(x + c * (x - xp));
-        }
+    // the faces on the upper boundary of each face family
+    long e = t - ncell;
+    const long nyz = (long)d.ny * d.nz, nxz = (long)d.nx * d.nz, nxy = (long)d.nx * d.ny;
+    if (e < nyz) {
+        const ptrdiff_t f = ix(d.nx + 1, d.ny, d.nx + 1, (int)(e % d.ny) + 1, (int)(e / d.ny) + 1);
+        s.xh[8][f] = float(ext(fb.bi, fb.bip, f));
+        return;
+    }
+    e -= nyz;
+    if (e < nxz) {
+        const ptrdiff_t f = ix(d.nx, d.ny + 1, (int)(e % d.nx) + 1, d.ny + 1, (int)(e / d.nx) + 1);
+        s.xh[9][f] = float(ext(fb.bj, fb.bjp, f));
+        return;
+    }
+    e -= nxz;
+    if (e < nxy) {
+        const ptrdiff_t f = ix(d.nx, d.ny, (int)(e % d.nx) + 1, (int)(e / d.nx) + 1, d.nz + 1);
+        s.xh[10][f] = float(ext(fb.bk, fb.bkp, f));
+    }
 }
 
 // ------------------------------------------------------------------------
