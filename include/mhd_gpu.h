@@ -160,6 +160,12 @@ int mhd_fetch(mhd_ctx *ctx, int which, double *host_out);
 int mhd_reconstruct(mhd_ctx *ctx, int dir, const double *f, int fn1, int fn2, int fn3, int lnx, int lny,
                     int lnz, double *fl, double *fr);
 
+/* divergence monitor (no counterpart in the reference, which has no div B
+ * diagnostic): *divb = max over the owned cells of |div B| of the current
+ * face field, each cell's net face flux over its volume as the update forms
+ * a flux divergence (metric included).  CT keeps it at round-off. */
+int mhd_divb_max(mhd_ctx *ctx, double *divb);
+
 /* diag_rho_neg / diag_overflow (mhd3d3.f90:3143-3160) plus p<=0:
  * *flags = 1 rho<0, 2 NaN interior rho, 4 interior p<=0 or NaN;
  * returns MHD_ERR_STATE when any flag is set */

@@ -1122,6 +1122,25 @@ int mhd_reconstruct(mhd_ctx* c, int dir, const double* f, int fn1, int fn2, int 
     return rc;
 }
 
+int mhd_divb_max(mhd_ctx* c, double* divb) {
+    if (!c || !divb || !c->have_geometry) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    const dim3 g = blocks(c->ninterior);
+    auto k = c->met.mode == 2 ? k_divb_partial<true> : k_divb_partial<false>;
+    // the CFL partials' buffer serves as scratch: the next get_dt recomputes them
+    launch(c, MHD_K_DT, g, dim3(kBlock), k, c->d, CFaces{c->bi, c->bj, c->bk}, sdx(c), sdy(c), sdz(c), gmet(c),
+           c->dtpart, c->ninterior);
+    c->dt_fresh = false;
+    launch(c, MHD_K_DT, dim3(1), dim3(1024), k_dt_final, (const double*)c->dtpart, (long)g.x, c->scal + SC_DIVB,
+           (double*)nullptr);
+    double h = 0.;
+    if (hipMemcpyAsync(&h, c->scal + SC_DIVB, sizeof(double), hipMemcpyDeviceToHost, c->s) != hipSuccess)
+        return MHD_ERR_HIP;
+    const int rc = sync_status(c);
+    *divb = -h;
+    return rc;
+}
+
 int mhd_check_state(mhd_ctx* c, int* flags) {
     if (!c) return MHD_ERR_PARAM;
     hipSetDevice(c->p.device);

@@ -32,8 +32,9 @@ struct Metric {
 };
 }  // namespace mhd
 
-// device scalars: scal[SC_TAU] = tau, scal[SC_TAU0] = tau0, scal[SC_DTCFL] = get_dt result
-enum { SC_TAU = 0, SC_TAU0 = 1, SC_DTCFL = 2 };
+// device scalars: scal[SC_TAU] = tau, scal[SC_TAU0] = tau0, scal[SC_DTCFL] = get_dt result,
+// scal[SC_DIVB] = -max |div B| of the last mhd_divb_max
+enum { SC_TAU = 0, SC_TAU0 = 1, SC_DTCFL = 2, SC_DIVB = 3 };
 
 struct SurrogateState;
 

@@ -463,6 +463,39 @@ __global__ void __launch_bounds__(kBlock) k_dt_partial(Dims d, double gam, doubl
 }
 
 // out[0] = min(part); if tau != null also tau[0] = out[0] (AB: tau from CFL)
+// divergence monitor: -|div B| of each interior cell, as the update forms a
+// flux divergence (tensor or affine metric: differences over the metric
+// spacing; general metric: face fluxes times areas over the volume); the
+// block minimum of the negated values goes to part[], k_dt_final then gives
+// -max |div B|
+template <bool GEN>
+__global__ void __launch_bounds__(kBlock) k_divb_partial(Dims d, CFaces b, const double* __restrict__ dx,
+                                                        const double* __restrict__ dy,
+                                                        const double* __restrict__ dz, GMet gm,
+                                                        double* __restrict__ part, long ncell) {
+    const long t = gtid();
+    double v = 0.;
+    if (t < ncell) {
+        int i, j, k;
+        unflat(t, d.nx0, d.ny0, i, j, k);
+        const int ng = d.ng;
+        const ptrdiff_t q = ix(d.nx, d.ny, ng + i, ng + j, ng + k);
+        const ptrdiff_t fi = ix(d.nx + 1, d.ny, ng + i, ng + j, ng + k), fj = ix(d.nx, d.ny + 1, ng + i, ng + j, ng + k);
+        const ptrdiff_t fk1 = q + (ptrdiff_t)d.nx * d.ny;
+        double div;
+        if constexpr (GEN)
+            div = ((b.bi[fi + 1] * gm.af[0][fi + 1] - b.bi[fi] * gm.af[0][fi]) +
+                   (b.bj[fj + d.nx] * gm.af[1][fj + d.nx] - b.bj[fj] * gm.af[1][fj]) +
+                   (b.bk[fk1] * gm.af[2][fk1] - b.bk[q] * gm.af[2][q])) * gm.ivol[q];
+        else
+            div = (b.bi[fi + 1] - b.bi[fi]) / dx[ng + i - 1] + (b.bj[fj + d.nx] - b.bj[fj]) / dy[ng + j - 1] +
+                  (b.bk[fk1] - b.bk[q]) / dz[ng + k - 1];
+        v = -fabs(div);
+    }
+    v = block_min(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
 __global__ void __launch_bounds__(1024) k_dt_final(const double* __restrict__ part, long npart,
                                                    double* __restrict__ out, double* __restrict__ tau) {
     __shared__ double red[16];

@@ -105,6 +105,12 @@ module mhd_gpu
          type(c_ptr), value :: ctx
          integer(c_int), intent(out) :: flags
       end function
+      ! max |div B| over the owned cells (no reference counterpart)
+      integer(c_int) function mhd_divb_max(ctx, divb) bind(C, name='mhd_divb_max')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: divb
+      end function
       integer(c_int) function mhd_synchronize(ctx) bind(C, name='mhd_synchronize')
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx

@@ -69,6 +69,7 @@ SIGNATURES = {
     "mhd_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]),
     "mhd_profile_classes": (C.c_int, [C.c_void_p, C.c_uint]),
     "mhd_set_stage_overlap": (C.c_int, [C.c_void_p, C.c_int]),
+    "mhd_divb_max": (C.c_int, [C.c_void_p, _D]),
     "mhd_ensemble_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int, _D]),
     "mhd_ensemble_step": (C.c_int, [C.c_void_p, C.c_int]),
     "mhd_ensemble_destroy": (None, [C.c_void_p]),
@@ -255,6 +256,12 @@ class Solver:
             _ck(lib().mhd_fetch(self.h, w, _p(a)), "mhd_fetch")
             out[name] = a
         return out
+
+    def divb_max(self):
+        """max |div B| over the owned cells (mhd_divb_max)"""
+        v = C.c_double(0.)
+        _ck(lib().mhd_divb_max(self.h, C.byref(v)), "mhd_divb_max")
+        return v.value
 
     def check_state(self):
         f = C.c_int(0)
