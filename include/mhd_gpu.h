@@ -209,7 +209,9 @@ int mhd_set_stage_overlap(mhd_ctx *ctx, int on);
  * primitives to the face's left/right states; it replaces PDM in the fp32
  * flux of a MHD_MIXED context created with surrogate = 1, with PDM as
  * fallback where the largest normalised second difference exceeds theta or
- * a density/pressure state is not positive.  Windows are centred on the
+ * a density/pressure state is not positive, and with the linear part
+ * alone where it is below 1e-3 (the gate keeps the net's correction within
+ * 1e-3 of the window scale there).  Windows are centred on the
  * mean of the two cells next to the face and divided by max(range, 1e-3 x
  * the variable's physical scale); states are clamped to the two cells'
  * range.

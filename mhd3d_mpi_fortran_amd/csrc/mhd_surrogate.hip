@@ -17,7 +17,8 @@
 //           4th-order interpolation (-x0 + 7 x1 + 7 x2 - x3)/12 of q's window
 //           and gate = the largest |second difference| of q's window, so the
 //           MLP learns PDM's correction to lin and the correction vanishes
-//           on linear data
+//           on linear data; in the solver, faces whose largest normalised
+//           second difference is at most ROUGH_MIN take lin alone
 //   parameters, packed: W1[32][40] b1[32] W2[32][32] b2[32] W3[16][32] b3[16]
 //
 // Kernels: sample collection from the solver's own state (binary64 PDM
@@ -85,6 +86,11 @@ __device__ __forceinline__ void var_scales(const A* c, A* sc) {
     sc[5] = sc[6] = sc[7] = b;
 }
 constexpr float NEGLIGIBLE = 1e-3f;
+// below this largest normalised second difference the face takes the linear
+// part alone: the gate bounds the net's correction by ROUGH_MIN times the
+// window scale there (windows that are constant up to rounding noise, or
+// linear to within 1e-3 of their range), so running the MLP buys nothing
+constexpr float ROUGH_MIN = 1e-3f;
 
 // normalisation of a window: s = max(range, 1e-3 * physical scale), so that
 // variations far below the variable's scale stay small after normalisation
@@ -538,11 +544,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
             G[v] = rough5(x + 5 * v);
             A[v] = lin4(x + 5 * v);
             smooth = smooth && (G[v] <= theta * ss[v]);
-            curved = curved || (G[v] > 0.f);
+            curved = curved || (G[v] > ROUGH_MIN * ss[v]);
         }
-        // the MLP only matters on faces that use the net and have a non-zero
-        // gate (all gates are zero on constant or linear windows, where
-        // y = lin exactly); MFMA needs the whole wave, so the test is per wave
+        // the MLP only runs for faces that use the net and whose windows are
+        // curved beyond ROUGH_MIN (all gates are zero on constant or linear
+        // windows, where y = lin exactly); MFMA needs the whole wave, so the
+        // test is per wave, and the other lanes of such a wave drop the MLP's
+        // output (y = lin, as in a wave without the MLP)
         float y[NO];
         const bool need = live && smooth && curved;
         if (__ballot(need)) {
@@ -563,6 +571,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
                 for (int a = 0; a < 5; ++a) x[5 * v + a] = (x[5 * v + a] - cc[v]) * r;
             }
             mlp_core(W, x, lane, y);
+            if (!need) {
+#pragma unroll
+                for (int o = 0; o < NO; ++o) y[o] = 0.f;
+            }
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
                 A[v] = stash[0][v][threadIdx.x];

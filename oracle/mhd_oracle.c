@@ -1039,6 +1039,8 @@ void orc_mlp_forward(const float *w, const float *x, long n, int h_round, float 
     }
 }
 
+#define SUR_ROUGH_MIN 1e-3
+
 /* the learned face states of one face (the solver's use of the net):
  * windows of cells j-2..j+2 centred on c = (f(j-1)+f(j))/2, divided by
  * max(range, 1e-3 * physical scale), net y = lin + MLP, clamped to the two
@@ -1084,6 +1086,9 @@ static int sur_face(const orc_state *s, const A3 *const *f, int dir, int i, int 
     } else {
         orc_mlp_forward(s->sur_w, xf, 1, 0, y);
     }
+    /* windows curved by at most SUR_ROUGH_MIN take the linear part alone */
+    if (!(rough > SUR_ROUGH_MIN))
+        for (int o = 0; o < MNO; ++o) y[o] = (float)mlin(xf, o);
     for (int v = 0; v < 8; ++v) {
         const double lo = fmin(xf[5 * v + 1], xf[5 * v + 2]), hi = fmax(xf[5 * v + 1], xf[5 * v + 2]);
         lv[v] = c[v] + ss[v] * fmin(hi, fmax(lo, (double)y[v]));
