@@ -76,6 +76,15 @@ struct VavWin {
             for (int q = 0; q < N; ++q) w[a][q] = w[a + 1][q];
         row(lv, ob + (unsigned)(N - 1 + LO) * osm, oso, w[N - 1]);
     }
+    // advance one cell with a row loaded beforehand
+    __device__ __forceinline__ void shift_in(const R* nxt) {
+#pragma unroll
+        for (int a = 0; a < N - 1; ++a)
+#pragma unroll
+            for (int q = 0; q < N; ++q) w[a][q] = w[a + 1][q];
+#pragma unroll
+        for (int q = 0; q < N; ++q) w[N - 1][q] = nxt[q];
+    }
 };
 
 // PDM: the d1-interpolation of footprint column m2 (offset m2-4 along d2)
@@ -156,13 +165,24 @@ __device__ __forceinline__ void vav_march(const TI* __restrict__ lv, ptrdiff_t b
 #pragma unroll
         for (int m2 = 0; m2 < 8; ++m2) g[m2] = pdm_col(W, m2);
     }
+    // fp32 UNIT windows (the fewest registers): the next step's row is
+    // loaded before the current corner velocity is computed, so its latency
+    // hides behind that arithmetic (binary64 windows would lose a wave per
+    // SIMD to the extra row)
+    constexpr int N = VavWin<R, M, MD1, UNIT>::N, LO = VavWin<R, M, MD1, UNIT>::LO;
+    constexpr bool PRE = UNIT && sizeof(R) == 4;
 #pragma unroll
     for (int s = 0; s < T; ++s) {
         if (s < cnt) {
+            R nxt[N];
+            if constexpr (PRE) {
+                if (s + 1 < cnt) W.row(lv, ob + osm + (unsigned)(N - 1 + LO) * osm, oso, nxt);
+            }
             out[s] = vav_win<R, M>(W, c, g);
             if (s + 1 < cnt) {
                 ob += osm;
-                W.shift(lv, ob, osm, oso);
+                if constexpr (PRE) W.shift_in(nxt);
+                else W.shift(lv, ob, osm, oso);
                 if constexpr (M == PDM) {
                     if constexpr (MD1) {
 #pragma unroll
