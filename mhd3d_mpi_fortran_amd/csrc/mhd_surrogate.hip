@@ -625,14 +625,12 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
 
 // PDM flux of the listed fallback faces (fp32, as the plain mixed path)
 template <int D, int MET>
-__global__ void __launch_bounds__(kBlock) k_flux_fix(Dims d, Coef<float> c, Prim<float> q, Normals<float> nm, GMet gm,
-                                                    const int* __restrict__ fix,
-                                                    const unsigned long long* __restrict__ nfix,
-                                                    float* __restrict__ F, long nface) {
-    const long n = (long)*nfix;
+__device__ __forceinline__ void flux_fix_dir(const Dims& d, const Coef<float>& c, const Prim<float>& q,
+                                             const Normals<float>& nm, const GMet& gm, const int* __restrict__ fix,
+                                             long n, float* __restrict__ F, long nface, long first, long stride) {
     const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
     const ptrdiff_t st = D == 1 ? 1 : (D == 2 ? (ptrdiff_t)d.nx : (ptrdiff_t)d.nx * d.ny);
-    for (long e = gtid(); e < n; e += (long)gridDim.x * blockDim.x) {
+    for (long e = first; e < n; e += stride) {
         const long t = fix[e];
         int i, j, k;
         unflat(t, ii, jj, i, j, k);
@@ -647,15 +645,32 @@ __global__ void __launch_bounds__(kBlock) k_flux_fix(Dims d, Coef<float> c, Prim
         recon_line<float, PDM>(q.by, base, st, c, l[6], r[6]);
         recon_line<float, PDM>(q.bz, base, st, c, l[7], r[7]);
         if (MET == 2) {
-            float n[3];
-            gen_normal<float, D>(d, gm, i, j, k, n);
-            face_flux_met<float, float>(c, l, r, n, F, t, nface);
+            float nn[3];
+            gen_normal<float, D>(d, gm, i, j, k, nn);
+            face_flux_met<float, float>(c, l, r, nn, F, t, nface);
         } else if (MET == 1) {
             face_flux_met<float, float>(c, l, r, nm.n[D - 1], F, t, nface);
         } else {
             face_flux<float, D>(c, l, r, F, t, nface);
         }
     }
+}
+
+// the three directions' fallback lists in one launch: blockIdx.y = direction
+struct FixLists {
+    const int* fix[3];
+    float* F[3];
+    long nface[3];
+};
+template <int MET>
+__global__ void __launch_bounds__(kBlock) k_flux_fix(Dims d, Coef<float> c, Prim<float> q, Normals<float> nm, GMet gm,
+                                                    FixLists fl, const unsigned long long* __restrict__ nfix) {
+    const long first = gtid(), stride = (long)gridDim.x * blockDim.x;
+    const int dir = blockIdx.y;
+    const long n = (long)nfix[dir];
+    if (dir == 0) flux_fix_dir<1, MET>(d, c, q, nm, gm, fl.fix[0], n, fl.F[0], fl.nface[0], first, stride);
+    else if (dir == 1) flux_fix_dir<2, MET>(d, c, q, nm, gm, fl.fix[1], n, fl.F[1], fl.nface[1], first, stride);
+    else flux_fix_dir<3, MET>(d, c, q, nm, gm, fl.fix[2], n, fl.F[2], fl.nface[2], first, stride);
 }
 
 __global__ void k_zero_u64(unsigned long long* __restrict__ p, int n) {
@@ -727,13 +742,10 @@ static void flux_sur_launch(mhd_ctx* c, SurrogateState* S, Src<float> s) {
     launch(c, MHD_K_FLUX, dim3(gsur[2]), dim3(kBlock), sur::k_flux_sur<3, MET>, c->d, co, q, nm, c->gm, theta, wbf, bias,
            (float*)c->flux[2], c->nf[2], fx[2], S->nfix + 2, st);
     // the fallback lists: a fixed grid walks whatever length each list has
-    const dim3 gfix(1024);
-    launch(c, MHD_K_SURR, gfix, dim3(kBlock), sur::k_flux_fix<1, MET>, c->d, co, q, nm, c->gm, (const int*)fx[0],
-           (const unsigned long long*)(S->nfix + 0), (float*)c->flux[0], c->nf[0]);
-    launch(c, MHD_K_SURR, gfix, dim3(kBlock), sur::k_flux_fix<2, MET>, c->d, co, q, nm, c->gm, (const int*)fx[1],
-           (const unsigned long long*)(S->nfix + 1), (float*)c->flux[1], c->nf[1]);
-    launch(c, MHD_K_SURR, gfix, dim3(kBlock), sur::k_flux_fix<3, MET>, c->d, co, q, nm, c->gm, (const int*)fx[2],
-           (const unsigned long long*)(S->nfix + 2), (float*)c->flux[2], c->nf[2]);
+    const sur::FixLists fl{{fx[0], fx[1], fx[2]}, {(float*)c->flux[0], (float*)c->flux[1], (float*)c->flux[2]},
+                      {c->nf[0], c->nf[1], c->nf[2]}};
+    launch(c, MHD_K_SURR, dim3(1024, 3), dim3(kBlock), sur::k_flux_fix<MET>, c->d, co, q, nm, c->gm, fl,
+           (const unsigned long long*)S->nfix);
 }
 
 void launch_flux_surrogate(mhd_ctx* c, Src<float> s) {
