@@ -221,6 +221,8 @@ int mhd_set_stage_overlap(mhd_ctx *ctx, int on);
 enum { MHD_SUR_NPARAM = 2896, MHD_SUR_SAMPLE = 56 };
 int mhd_surrogate_set_weights(mhd_ctx *ctx, const float *w, int n);
 int mhd_surrogate_get_weights(mhd_ctx *ctx, float *w, int n);
+/* theta: smoothness threshold of the net (finite; theta < 0 sends every
+ * face to the PDM fallback) */
 int mhd_surrogate_configure(mhd_ctx *ctx, double theta, int count_fallbacks);
 /* faces evaluated / PDM fallbacks since the last call (count_fallbacks on) */
 int mhd_surrogate_stats(mhd_ctx *ctx, long *faces, long *fallbacks);

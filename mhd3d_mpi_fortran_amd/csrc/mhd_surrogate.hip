@@ -799,7 +799,8 @@ int mhd_surrogate_get_weights(mhd_ctx* c, float* w, int n) {
 }
 
 int mhd_surrogate_configure(mhd_ctx* c, double theta, int count_fallbacks) {
-    if (!c || !(theta >= 0.)) return MHD_ERR_PARAM;
+    // theta < 0: every face takes the PDM fallback
+    if (!c || !std::isfinite(theta)) return MHD_ERR_PARAM;
     c->sur_theta = (float)theta;
     c->sur_stats = count_fallbacks != 0;
     return MHD_OK;

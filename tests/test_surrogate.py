@@ -311,3 +311,38 @@ def test_solver_with_learned_reconstruction(trained):
     # correction must keep the solver at that level
     assert l1["net"] <= 1e-4
     assert l1["net"] <= 2. * l1["linear"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("grid", ["cartesian", "sheared", "shell"])
+def test_all_fallback_equals_mixed_pdm(grid):
+    """theta < 0 sends every face to the PDM fallback list: the learned path
+    (flux kernel, one-launch fix-up of the three lists) must then reproduce
+    the mixed-precision PDM path bit for bit, on the reference's grid, the
+    affine sheared grid and the spherical shell's node grid"""
+    from mhd3d_mpi_fortran_amd import shell
+    if grid == "shell":
+        g = shell.spherical_shell(16, 16, 24, ng=4, theta0=.6)
+        st = shell.dipole_inflow(g, 9)
+        shp, bc = (16, 16, 24), ("even", "even", "periodic")
+    else:
+        A = [[1., .5, 0.], [0., 1., .2], [0., 0., 1.]] if grid == "sheared" else None
+        g = ics.Grid(20, 16, 12, ng=4, A=A)
+        st = ics.initial_state(g, "blast", 13, radius=.2)
+        shp, bc = (20, 16, 12), "periodic"
+    out = []
+    for sur in (0, 1):
+        s = gpu.Solver(gpu.make_params(*shp, ng=4, reconop=1, stepop=1, bc=bc, precision="mixed", surrogate=sur), g)
+        if sur:
+            sg.set_weights(s, _weights(11, 1.))
+            sg.configure(s, -1., count=True)
+        s.upload(st)
+        tau = s.initialize(1e-4)
+        s.step(6, tau)
+        out.append(s.download())
+        if sur:
+            faces, fb = sg.stats(s)
+            assert faces > 0 and fb == faces
+        s.close()
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k
