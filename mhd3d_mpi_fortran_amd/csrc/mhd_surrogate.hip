@@ -539,8 +539,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
         for (int v = 0; v < 8; ++v) {
             ss[v] = norm_scale<float>(mm[v], sc[v]);
             const float w1 = x[5 * v + 1], w2 = x[5 * v + 2];
-            lo[v] = fminf(w1, w2);
-            hi[v] = fmaxf(w1, w2);
+            lo[v] = w1 < w2 ? w1 : w2;   // (a NaN window fails the smoothness test below)
+            hi[v] = w1 < w2 ? w2 : w1;
             G[v] = rough5(x + 5 * v);
             A[v] = lin4(x + 5 * v);
             smooth = smooth && (G[v] <= theta * ss[v]);
@@ -589,8 +589,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
         float lv[8], rv[8];
 #pragma unroll
         for (int v = 0; v < 8; ++v) {
-            lv[v] = fminf(hi[v], fmaxf(lo[v], A[v] + G[v] * y[v]));
-            rv[v] = fminf(hi[v], fmaxf(lo[v], A[v] + G[v] * y[8 + v]));
+            // clamp to [lo, hi] (lo <= hi): the median of the three
+            lv[v] = __builtin_amdgcn_fmed3f(lo[v], hi[v], A[v] + G[v] * y[v]);
+            rv[v] = __builtin_amdgcn_fmed3f(lo[v], hi[v], A[v] + G[v] * y[8 + v]);
         }
         const bool fb = !smooth || !(lv[0] > 0.f) || !(rv[0] > 0.f) || !(lv[4] > 0.f) || !(rv[4] > 0.f);
         // fallback faces go to a list that k_flux_fix evaluates with PDM
