@@ -193,11 +193,12 @@ int mhd_profile_read(mhd_ctx *ctx, int kclass, int *launches, double *ms);
  * needs */
 int mhd_profile_classes(mhd_ctx *ctx, unsigned mask);
 
-/* stage scheduling (no counterpart in the reference): on (the default), a
- * stage's edge EMF and CT run on a second stream beside its face flux and
- * cell update; off, every kernel runs on the context's stream in the
- * reference's order (the per-kernel times then add up to the step).  The
- * results are the same bit for bit either way. */
+/* stage scheduling (no counterpart in the reference): on (the default for
+ * blocks of at least 2^21 owned cells), a stage's edge EMF and CT run on a
+ * second stream beside its face flux and cell update; off (the default for
+ * smaller blocks, and for ensemble members), every kernel runs on the
+ * context's stream in the reference's order (the per-kernel times then add
+ * up to the step).  The results are the same bit for bit either way. */
 int mhd_set_stage_overlap(mhd_ctx *ctx, int on);
 
 /* ---- learned face reconstruction (no counterpart in the reference) ------

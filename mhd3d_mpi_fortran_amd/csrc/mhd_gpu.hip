@@ -95,6 +95,9 @@ int mhd_create(mhd_ctx** out, const mhd_params* p) {
     c->ne[2] = (long)(d.nx0 + 1) * (d.ny0 + 1) * d.nz0;
     c->ninterior = (long)d.nx0 * d.ny0 * d.nz0;
     c->emf_fill = kEmfFillThreads;
+    // the two-stream stage pays for its fork and join only on blocks whose
+    // kernels fill the chip for long; small blocks keep one stream
+    c->overlap = c->ninterior >= kOverlapCells;
     if (const char* ev = getenv("MHD_EMF_FILL_THREADS")) c->emf_fill = atol(ev);
     long nmax = c->ncell;
     nmax = c->nbi > nmax ? c->nbi : nmax;
@@ -1004,10 +1007,12 @@ int mhd_ensemble_create(mhd_ens** out, mhd_ctx* const* members, int n, const dou
         mhd_ensemble_destroy(e);
         return MHD_ERR_HIP;
     }
-    std::vector<bool> prof(n);
+    std::vector<bool> prof(n), ovl(n);
     for (int a = 0; a < n; ++a) {
         prof[a] = e->m[a]->prof;
-        e->m[a]->prof = false;   // no event records inside the graph
+        ovl[a] = e->m[a]->overlap;
+        e->m[a]->prof = false;      // no event records inside the graph
+        e->m[a]->overlap = false;   // the members' branches already run side by side
         e->m[a]->tau0_host = tau[a];
     }
     ok = hipStreamBeginCapture(e->root, hipStreamCaptureModeThreadLocal) == hipSuccess &&
@@ -1026,6 +1031,7 @@ int mhd_ensemble_create(mhd_ens** out, mhd_ctx* const* members, int n, const dou
     e->graph = g;
     for (int a = 0; a < n; ++a) {
         e->m[a]->prof = prof[a];
+        e->m[a]->overlap = ovl[a];
         e->m[a]->launch_error = hipSuccess;
     }
     ok = ok && ended && g && hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0) == hipSuccess;

@@ -42,6 +42,9 @@ struct SurrogateState;
 // (fewer than ~8 per SIMD: 256 CUs x 4 SIMDs x 8 waves x 64 lanes) take
 // short runs
 constexpr long kEmfFillThreads = 256L * 4 * 8 * 64;
+// blocks with at least this many owned cells run a stage on two streams by
+// default (mhd_set_stage_overlap)
+constexpr long kOverlapCells = 1L << 21;
 
 struct mhd_ctx {
     mhd_params p{};
