@@ -58,8 +58,12 @@ struct VavWin {
 #pragma unroll
             for (int q = 0; q < N; ++q) out[q] = R(p[q + LO]);
         } else {
+            // the column offsets are uniform: they go into each load's
+            // scalar base, so a row needs one offset register, not N
+            const char* b = reinterpret_cast<const char*>(lv);
 #pragma unroll
-            for (int q = 0; q < N; ++q) out[q] = R(load_at(lv, orow + (unsigned)(q + LO) * oso));
+            for (int q = 0; q < N; ++q)
+                out[q] = R(load_at(reinterpret_cast<const TI*>(b + (ptrdiff_t)(q + LO) * (ptrdiff_t)oso), orow));
         }
     }
     template <typename TI>
@@ -289,16 +293,18 @@ __global__ void __launch_bounds__(kBlock) k_emf_march(Dims d, Coef<R> c, EmfIn<T
             const R b1u = (R(1) + sig2) / R(2) * b1l + (R(1) - sig2) / R(2) * b1r;
             const R b2u = (R(1) + sig1) / R(2) * b2l + (R(1) - sig1) / R(2) * b2r;
             R dum = -v1av[s] * b2u + v2av[s] * b1u;
+            // the four cells around the edge: uniform offsets in the scalar bases
+            auto vfat = [&](unsigned back) {
+                return R(load_at(reinterpret_cast<const TI*>(reinterpret_cast<const char*>(in.vf) - (ptrdiff_t)back),
+                                 obc));
+            };
             R eta;
             if (K == 1)
-                eta = R(.25) * (R(load_at(in.vf, obc - osy - osz)) + R(load_at(in.vf, obc - osz)) +
-                                R(load_at(in.vf, obc)) + R(load_at(in.vf, obc - osy)));
+                eta = R(.25) * (vfat(osy + osz) + vfat(osz) + vfat(0u) + vfat(osy));
             else if (K == 2)
-                eta = R(.25) * (R(load_at(in.vf, obc - osx - osz)) + R(load_at(in.vf, obc - osx)) +
-                                R(load_at(in.vf, obc)) + R(load_at(in.vf, obc - osz)));
+                eta = R(.25) * (vfat(osx + osz) + vfat(osx) + vfat(0u) + vfat(osz));
             else
-                eta = R(.25) * (R(load_at(in.vf, obc - osx - osy)) + R(load_at(in.vf, obc - osy)) +
-                                R(load_at(in.vf, obc)) + R(load_at(in.vf, obc - osx)));
+                eta = R(.25) * (vfat(osx + osy) + vfat(osy) + vfat(0u) + vfat(osx));
             dum = dum + eta * (b1l + b2r - b1r - b2l);
             // ef = |C_a||C_b|/det A of an affine metric (E . a_k per unit
             // computational length), exactly 1 on the reference's grid; a

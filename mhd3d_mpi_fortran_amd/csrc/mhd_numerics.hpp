@@ -251,8 +251,11 @@ __device__ __forceinline__ void recon_off(const T* __restrict__ f, unsigned ob, 
 #pragma unroll
         for (int m = 0; m < S::n; ++m) s[m] = R(p[S::lo + m]);
     } else {
+        // uniform tap offsets in each load's scalar base: one offset register
+        const char* b = reinterpret_cast<const char*>(f);
 #pragma unroll
-        for (int m = 0; m < S::n; ++m) s[m] = R(load_at(f, ob + (unsigned)(S::lo + m) * ost));
+        for (int m = 0; m < S::n; ++m)
+            s[m] = R(load_at(reinterpret_cast<const T*>(b + (ptrdiff_t)(S::lo + m) * (ptrdiff_t)ost), ob));
     }
     S::run(s, c, fl, fr);
 }
