@@ -62,6 +62,33 @@ __device__ __forceinline__ unsigned xcd_block() {
 }
 __device__ __forceinline__ long gtid_xcd() { return (long)xcd_block() * blockDim.x + threadIdx.x; }
 
+// Face tiles of the strided sweeps (D = 2, 3).  A workgroup's 256 points are
+// 64 consecutive faces along x (one wavefront) in 4 consecutive planes along
+// the sweep, so the four wavefronts' tap rows overlap -- a W-tap window
+// touches W+3 rows per array instead of 4W -- and repeat in the CU's L1
+// rather than in L2.  Tiles run x-fastest, then along the sweep, then across
+// it, so a run of consecutive tiles on one XCD also shares its rows in L2.
+// (ii, jj, kk) = the direction's face counts.
+constexpr int kTileX = 64, kTileN = kBlock / kTileX;
+template <int D>
+__host__ __device__ __forceinline__ unsigned face_tiles(int ii, int jj, int kk) {
+    const unsigned nn = D == 2 ? jj : kk, no = D == 2 ? kk : jj;
+    return (unsigned)((ii + kTileX - 1) / kTileX) * ((nn + kTileN - 1) / kTileN) * no;
+}
+// 1-based face (i, j, k) of the calling thread in tile `tile` (the
+// convention of unflat and ix); false past the edges
+template <int D>
+__device__ __forceinline__ bool tile_face(unsigned tile, int ii, int jj, int kk, int& i, int& j, int& k) {
+    const unsigned nn = D == 2 ? jj : kk, no = D == 2 ? kk : jj;
+    const unsigned ntx = (ii + kTileX - 1) / kTileX, ntn = (nn + kTileN - 1) / kTileN;
+    const unsigned tx = tile % ntx, r = tile / ntx, tn = r % ntn, o = r / ntn;
+    const unsigned x = tx * kTileX + (threadIdx.x & (kTileX - 1)), s = tn * kTileN + threadIdx.x / kTileX;
+    i = (int)x + 1;
+    j = (int)(D == 2 ? s : o) + 1;
+    k = (int)(D == 2 ? o : s) + 1;
+    return x < (unsigned)ii && s < nn && o < no;
+}
+
 template <typename T>
 struct Prim {  // cell-centred state read by the flux and EMF kernels
     const T *rho, *vx, *vy, *vz, *p, *bx, *by, *bz;
@@ -225,11 +252,17 @@ __device__ __forceinline__ void gen_normal(const Dims& d, const GMet& g, int i, 
 template <typename R, int M, int D, typename TI, typename TF, int MET>
 __global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<TI> q, Normals<R> nm, GMet gm,
                                                  TF* __restrict__ F, long nface) {
-    const long t = gtid_xcd();
-    if (t >= nface) return;
-    const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
+    const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2), kk = d.nz0 + (D == 3);
     int i, j, k;
-    unflat(t, ii, jj, i, j, k);
+    long t;
+    if constexpr (D == 1) {  // flat: the x sweep's taps are contiguous
+        t = gtid_xcd();
+        if (t >= nface) return;
+        unflat(t, ii, jj, i, j, k);
+    } else {
+        if (!tile_face<D>(xcd_block(), ii, jj, kk, i, j, k)) return;
+        t = ix(ii, jj, i, j, k);
+    }
     const ptrdiff_t base = ix(d.nx, d.ny, d.ng + i, d.ng + j, d.ng + k);
     const ptrdiff_t st = D == 1 ? 1 : (D == 2 ? (ptrdiff_t)d.nx : (ptrdiff_t)d.nx * d.ny);
     R l[8], r[8];

@@ -15,10 +15,11 @@ void flux_launch_mm(mhd_ctx* c, Src<TI> s) {
     const Normals<R> nm = make_normals<R>(c);
     launch(c, MHD_K_FLUX, blocks(c->nf[0]), dim3(kBlock), k_flux<R, M, 1, TI, TF, MET>, c->d, co, q, nm, c->gm,
            (TF*)c->flux[0], c->nf[0]);
-    launch(c, MHD_K_FLUX, blocks(c->nf[1]), dim3(kBlock), k_flux<R, M, 2, TI, TF, MET>, c->d, co, q, nm, c->gm,
-           (TF*)c->flux[1], c->nf[1]);
-    launch(c, MHD_K_FLUX, blocks(c->nf[2]), dim3(kBlock), k_flux<R, M, 3, TI, TF, MET>, c->d, co, q, nm, c->gm,
-           (TF*)c->flux[2], c->nf[2]);
+    const Dims& d = c->d;
+    launch(c, MHD_K_FLUX, dim3(face_tiles<2>(d.nx0, d.ny0 + 1, d.nz0)), dim3(kBlock), k_flux<R, M, 2, TI, TF, MET>,
+           c->d, co, q, nm, c->gm, (TF*)c->flux[1], c->nf[1]);
+    launch(c, MHD_K_FLUX, dim3(face_tiles<3>(d.nx0, d.ny0, d.nz0 + 1)), dim3(kBlock), k_flux<R, M, 3, TI, TF, MET>,
+           c->d, co, q, nm, c->gm, (TF*)c->flux[2], c->nf[2]);
 }
 
 template <typename R, typename TI, typename TF, int M>

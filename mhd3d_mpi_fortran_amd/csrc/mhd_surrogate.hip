@@ -477,8 +477,9 @@ __global__ void __launch_bounds__(kBlock) k_infer(const float* __restrict__ x, l
 
 // ---------------------------------------------------------------------------
 // fp32 face flux with the learned reconstruction; each block covers
-// SUR_GROUPS x 256 consecutive faces, so the weights staged in LDS serve
-// several groups.  Faces that fail the smoothness or positivity check are
+// SUR_GROUPS groups of 256 faces (consecutive along x, face tiles in the
+// strided sweeps), so the weights staged in LDS serve several groups.
+// Faces that fail the smoothness or positivity check are
 // appended to a list (one atomic per wavefront) and get the PDM flux from
 // k_flux_fix, which keeps PDM's registers out of this kernel.
 // ---------------------------------------------------------------------------
@@ -496,15 +497,26 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
     load_net(W, wbf, bias);
     const ptrdiff_t st = D == 1 ? 1 : (D == 2 ? (ptrdiff_t)d.nx : (ptrdiff_t)d.nx * d.ny);
     const float* f[8] = {q.rho, q.vx, q.vy, q.vz, q.p, q.bx, q.by, q.bz};
-    const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
+    const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2), kk = d.nz0 + (D == 3);
     for (int grp = 0; grp < SUR_GROUPS; ++grp) {
-        const long t = ((long)xcd_block<kXcdRun / SUR_GROUPS>() * SUR_GROUPS + grp) * blockDim.x + threadIdx.x;
-        const bool live = t < nface;
-        if (__ballot(live) == 0ull) break;  // wave-uniform: later groups are past the end too
-        // lanes past the end read the first face's windows (their results
-        // are never written), so the loads need no branches
+        // groups of 256 faces: flat for the x sweep, face tiles for the
+        // strided sweeps.  Lanes past the end read the first face's windows
+        // (their results are never written), so the loads need no branches.
+        const unsigned g = xcd_block<kXcdRun / SUR_GROUPS>() * SUR_GROUPS + grp;
         int i, j, k;
-        unflat(live ? t : 0, ii, jj, i, j, k);
+        long t;
+        bool live;
+        if constexpr (D == 1) {
+            t = (long)g * blockDim.x + threadIdx.x;
+            live = t < nface;
+            if (__ballot(live) == 0ull) break;  // wave-uniform: later groups are past the end too
+            unflat(live ? t : 0, ii, jj, i, j, k);
+        } else {
+            live = tile_face<D>(g, ii, jj, kk, i, j, k);
+            if (__ballot(live) == 0ull) continue;  // a tile's last planes can lie past the sweep's end
+            if (!live) i = j = k = 1;
+            t = ix(ii, jj, i, j, k);
+        }
         // 32-bit byte offsets of the 5 taps from each array's uniform base
         // (every array stays below 2^32 bytes, checked at mhd_create)
         const unsigned base = (unsigned)ix(d.nx, d.ny, d.ng + i, d.ng + j, d.ng + k);
@@ -731,9 +743,11 @@ static void flux_sur_launch(mhd_ctx* c, SurrogateState* S, Src<float> s) {
     unsigned long long* st = c->sur_stats ? S->stats : nullptr;
     launch(c, MHD_K_SURR, dim3(1), dim3(64), sur::k_zero_u64, S->nfix, 3);
     int* fx[3] = {S->fix, S->fix + c->nf[0], S->fix + c->nf[0] + c->nf[1]};
-    const unsigned gsur[3] = {(unsigned)((c->nf[0] + sur::SUR_GROUPS * kBlock - 1) / (sur::SUR_GROUPS * kBlock)),
-                              (unsigned)((c->nf[1] + sur::SUR_GROUPS * kBlock - 1) / (sur::SUR_GROUPS * kBlock)),
-                              (unsigned)((c->nf[2] + sur::SUR_GROUPS * kBlock - 1) / (sur::SUR_GROUPS * kBlock))};
+    const Dims& d = c->d;
+    const unsigned sg = sur::SUR_GROUPS;
+    const unsigned gsur[3] = {(unsigned)((c->nf[0] + sg * kBlock - 1) / (sg * kBlock)),
+                              (face_tiles<2>(d.nx0, d.ny0 + 1, d.nz0) + sg - 1) / sg,
+                              (face_tiles<3>(d.nx0, d.ny0, d.nz0 + 1) + sg - 1) / sg};
     const unsigned short* wbf = S->wbf;
     const float* bias = S->bias;
     launch(c, MHD_K_FLUX, dim3(gsur[0]), dim3(kBlock), sur::k_flux_sur<1, MET>, c->d, co, q, nm, c->gm, theta, wbf, bias,
