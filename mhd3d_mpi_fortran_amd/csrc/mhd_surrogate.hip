@@ -26,6 +26,7 @@
 // reduction, Adam) and bf16-MFMA inference fused into the fp32 face flux.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -128,9 +129,10 @@ struct State {
     unsigned short* wbf = nullptr;  // bf16 MFMA A operands per lane (k_pack_bf16)
     float* bias = nullptr;     // biases in D-register order per half-wave (fp32)
     unsigned long long* stats = nullptr;  // faces evaluated, fallbacks
-    int* fix = nullptr;                   // fallback face lists of the three directions
-    long fix_n = 0;
-    unsigned long long* nfix = nullptr;   // their lengths
+    int* fix = nullptr;                   // fallback-list segments of the three directions
+    int* net = nullptr;                   // net-list segments
+    unsigned long long* ctr = nullptr;    // the segments' lengths
+    long seg_n = 0;                       // capacity in segments (all directions)
     long step = 0;
     bool have_weights = false;
 };
@@ -476,33 +478,131 @@ __global__ void __launch_bounds__(kBlock) k_infer(const float* __restrict__ x, l
 }
 
 // ---------------------------------------------------------------------------
-// fp32 face flux with the learned reconstruction; each block covers
-// SUR_GROUPS groups of 256 faces (consecutive along x, face tiles in the
-// strided sweeps), so the weights staged in LDS serve several groups.
-// Faces that fail the smoothness or positivity check are
-// appended to a list (one atomic per wavefront) and get the PDM flux from
-// k_flux_fix, which keeps PDM's registers out of this kernel.
+// fp32 face flux with the learned reconstruction, in two passes.
+//
+// Pass 1 (k_flux_sur) loads every face's 8 windows and takes what needs no
+// net: the faces whose windows are flat to within ROUGH_MIN (all gates are
+// zero there, y = lin) get the clamped linear part and their flux; faces
+// that fail the smoothness or positivity check are appended to a PDM
+// fallback list (k_flux_fix); faces that use the net are appended to a
+// net list.  Pass 2 (k_flux_mlp) runs the bf16 MFMA MLP over the net list,
+// 64 listed faces to a wavefront: MFMA needs the whole wavefront, and the
+// faces that use the net (about 1 % at 256^3) are scattered thinly over the
+// faces, so evaluating them in place would run the net in wavefronts where
+// most lanes discard it.  Lists are appended with one atomic per wavefront.
+// Atomics on one address serialise, and a tenth of the wavefronts hold a
+// net face (a few hundredths a fallback face), so both kinds of list come in
+// segments: the faces of kNetSeg consecutive pass-1 workgroups, with room
+// for all of them and a counter per list on a 128-byte line of its own.
+// The later passes give each segment one workgroup.
 // ---------------------------------------------------------------------------
 constexpr int SUR_GROUPS = 4;
+constexpr int kNetSeg = 16, kNetSegCap = kNetSeg * SUR_GROUPS * kBlock;
+// counters of segment s: [kCtrStride s + kCtrNet] net list, [.. + kCtrFix] fallback list
+constexpr int kCtrStride = 32, kCtrNet = 0, kCtrFix = 16;
+
+// the 8 windows (5 taps each, physical units) of face (i, j, k) along D
+template <int D>
+__device__ __forceinline__ void face_windows(const Dims& d, const float* const* f, int i, int j, int k, float* x) {
+    const ptrdiff_t st = D == 1 ? 1 : (D == 2 ? (ptrdiff_t)d.nx : (ptrdiff_t)d.nx * d.ny);
+    // 32-bit byte offsets of the 5 taps from each array's uniform base
+    // (every array stays below 2^32 bytes, checked at mhd_create)
+    const unsigned base = (unsigned)ix(d.nx, d.ny, d.ng + i, d.ng + j, d.ng + k);
+    unsigned off[5];
+#pragma unroll
+    for (int a = 0; a < 5; ++a) off[a] = (base + (unsigned)((a - 2) * st)) * 4u;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        if constexpr (D == 1) {  // contiguous taps: wide loads from one address
+#pragma unroll
+            for (int a = 0; a < 5; ++a) x[5 * v + a] = f[v][(ptrdiff_t)base + a - 2];
+        } else {
+#pragma unroll
+            for (int a = 0; a < 5; ++a) x[5 * v + a] = load_at(f[v], off[a]);
+        }
+    }
+}
+
+// per variable, in physical units: the window centre cc and normalisation
+// scale ss, the linear face value A = lin of the window (the 4th-order
+// interpolation commutes with the affine normalisation: its weights sum to
+// 1), the gate times the window scale G = the largest |second difference|
+// of the window, and the clamp range [lo, hi] of the two cells next to the
+// face (PDM's own bound, mhd3d3.f90:1142-1166).  The smoothness test
+// rough <= theta on the normalised window is G <= theta * ss; curved = some
+// G > ROUGH_MIN * ss.  Both passes derive these the same way.
+__device__ __forceinline__ void face_parts(const float* x, float theta, float* cc, float* ss, float* A, float* G,
+                                           float* lo, float* hi, bool& smooth, bool& curved) {
+    float mm[8], sc[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) window_stats(x + 5 * v, cc[v], mm[v]);
+    var_scales<float>(cc, sc);
+    smooth = true;
+    curved = false;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        ss[v] = norm_scale<float>(mm[v], sc[v]);
+        const float w1 = x[5 * v + 1], w2 = x[5 * v + 2];
+        lo[v] = w1 < w2 ? w1 : w2;   // (a NaN window fails the smoothness test)
+        hi[v] = w1 < w2 ? w2 : w1;
+        G[v] = rough5(x + 5 * v);
+        A[v] = lin4(x + 5 * v);
+        smooth = smooth && (G[v] <= theta * ss[v]);
+        curved = curved || (G[v] > ROUGH_MIN * ss[v]);
+    }
+}
+
+// append the lanes set in `m` (a ballot) to list[*n ...], one atomic per wave
+__device__ __forceinline__ void list_append(unsigned long long m, int lane, long t, int* __restrict__ list,
+                                            unsigned long long* __restrict__ n) {
+    if (!m) return;
+    unsigned long long first = 0;
+    const int leader = __ffsll((long long)m) - 1;
+    if (lane == leader) first = atomicAdd(n, (unsigned long long)__popcll(m));
+    first = __shfl(first, leader, 64);
+    if ((m >> lane) & 1ull) list[first + __popcll(m & ((1ull << lane) - 1ull))] = (int)t;
+}
+
+// the flux of face t from its (clamped) face values
+template <int D, int MET>
+__device__ __forceinline__ void sur_face_flux(const Dims& d, const Coef<float>& c, const Normals<float>& nm,
+                                              const GMet& gm, int i, int j, int k, const float* lv, const float* rv,
+                                              float* __restrict__ F, long t, long nface) {
+    if (MET == 2) {
+        float n[3];
+        gen_normal<float, D>(d, gm, i, j, k, n);
+        face_flux_met<float, float>(c, lv, rv, n, F, t, nface);
+    } else if (MET == 1) {
+        face_flux_met<float, float>(c, lv, rv, nm.n[D - 1], F, t, nface);
+    } else {
+        face_flux<float, D>(c, lv, rv, F, t, nface);
+    }
+}
+
+// positivity of density and pressure on both sides
+__device__ __forceinline__ bool sur_positive(const float* lv, const float* rv) {
+    return lv[0] > 0.f && rv[0] > 0.f && lv[4] > 0.f && rv[4] > 0.f;
+}
 
 template <int D, int MET>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_flux_sur(Dims d, Coef<float> c, Prim<float> q, Normals<float> nm, GMet gm, float theta,
-                                                    const unsigned short* __restrict__ wbf,
-                                                    const float* __restrict__ bias, float* __restrict__ F, long nface,
-                                                    int* __restrict__ fix, unsigned long long* __restrict__ nfix,
+__global__ void __launch_bounds__(kBlock) k_flux_sur(Dims d, Coef<float> c, Prim<float> q, Normals<float> nm, GMet gm,
+                                                    float theta, float* __restrict__ F, long nface,
+                                                    int* __restrict__ fix, int* __restrict__ net,
+                                                    unsigned long long* __restrict__ ctr,
                                                     unsigned long long* __restrict__ stats) {
     const int lane = threadIdx.x & 63;
-    __shared__ NetW W;
-    __shared__ float stash[4][8][kBlock];
-    load_net(W, wbf, bias);
-    const ptrdiff_t st = D == 1 ? 1 : (D == 2 ? (ptrdiff_t)d.nx : (ptrdiff_t)d.nx * d.ny);
     const float* f[8] = {q.rho, q.vx, q.vy, q.vz, q.p, q.bx, q.by, q.bz};
     const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2), kk = d.nz0 + (D == 3);
+    const unsigned lb = xcd_block<kXcdRun / SUR_GROUPS>();
+    // this workgroup's list segments and their counters
+    int* const sfix = fix + (size_t)(lb / kNetSeg) * kNetSegCap;
+    int* const snet = net + (size_t)(lb / kNetSeg) * kNetSegCap;
+    unsigned long long* const sc = ctr + (size_t)(lb / kNetSeg) * kCtrStride;
     for (int grp = 0; grp < SUR_GROUPS; ++grp) {
         // groups of 256 faces: flat for the x sweep, face tiles for the
         // strided sweeps.  Lanes past the end read the first face's windows
         // (their results are never written), so the loads need no branches.
-        const unsigned g = xcd_block<kXcdRun / SUR_GROUPS>() * SUR_GROUPS + grp;
+        const unsigned g = lb * SUR_GROUPS + grp;
         int i, j, k;
         long t;
         bool live;
@@ -517,104 +617,22 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
             if (!live) i = j = k = 1;
             t = ix(ii, jj, i, j, k);
         }
-        // 32-bit byte offsets of the 5 taps from each array's uniform base
-        // (every array stays below 2^32 bytes, checked at mhd_create)
-        const unsigned base = (unsigned)ix(d.nx, d.ny, d.ng + i, d.ng + j, d.ng + k);
-        unsigned off[5];
-#pragma unroll
-        for (int a = 0; a < 5; ++a) off[a] = (base + (unsigned)((a - 2) * st)) * 4u;
-        // windows in physical units
-        float x[NI], cc[8], mm[8], sc[8];
-#pragma unroll
-        for (int v = 0; v < 8; ++v) {
-            if constexpr (D == 1) {  // contiguous taps: wide loads from one address
-#pragma unroll
-                for (int a = 0; a < 5; ++a) x[5 * v + a] = f[v][(ptrdiff_t)base + a - 2];
-            } else {
-#pragma unroll
-                for (int a = 0; a < 5; ++a) x[5 * v + a] = load_at(f[v], off[a]);
-            }
-            window_stats(x + 5 * v, cc[v], mm[v]);
-        }
-        var_scales<float>(cc, sc);
-        // per variable, in physical units: the linear face value A = lin of
-        // the window (the 4th-order interpolation commutes with the affine
-        // normalisation: its weights sum to 1), the gate times the window
-        // scale G = the largest |second difference| of the window, and the
-        // clamp range [lo, hi] of the two cells next to the face (PDM's own
-        // bound, mhd3d3.f90:1142-1166).  The smoothness test rough <= theta
-        // on the normalised window is G <= theta * s; the normalisation
-        // itself is only needed as the MLP's input.
-        float A[8], G[8], lo[8], hi[8], ss[8];
-        bool smooth = true, curved = false;
-#pragma unroll
-        for (int v = 0; v < 8; ++v) {
-            ss[v] = norm_scale<float>(mm[v], sc[v]);
-            const float w1 = x[5 * v + 1], w2 = x[5 * v + 2];
-            lo[v] = w1 < w2 ? w1 : w2;   // (a NaN window fails the smoothness test below)
-            hi[v] = w1 < w2 ? w2 : w1;
-            G[v] = rough5(x + 5 * v);
-            A[v] = lin4(x + 5 * v);
-            smooth = smooth && (G[v] <= theta * ss[v]);
-            curved = curved || (G[v] > ROUGH_MIN * ss[v]);
-        }
-        // the MLP only runs for faces that use the net and whose windows are
-        // curved beyond ROUGH_MIN (all gates are zero on constant or linear
-        // windows, where y = lin exactly); MFMA needs the whole wave, so the
-        // test is per wave, and the other lanes of such a wave drop the MLP's
-        // output (y = lin, as in a wave without the MLP)
-        float y[NO];
+        float x[NI];
+        face_windows<D>(d, f, i, j, k, x);
+        float cc[8], ss[8], A[8], G[8], lo[8], hi[8];
+        bool smooth, curved;
+        face_parts(x, theta, cc, ss, A, G, lo, hi, smooth, curved);
+        // faces that use the net (smooth) and are curved beyond ROUGH_MIN go
+        // to the net list; the others that use the net take y = 0, the
+        // clamped linear part
         const bool need = live && smooth && curved;
-        if (__ballot(need)) {
-            // A, G and the clamp range wait in LDS (each thread its own
-            // column) while the MLP holds the registers
-#pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                stash[0][v][threadIdx.x] = A[v];
-                stash[1][v][threadIdx.x] = G[v];
-                stash[2][v][threadIdx.x] = lo[v];
-                stash[3][v][threadIdx.x] = hi[v];
-            }
-            // the net's inputs: windows centred on c and divided by s
-#pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                const float r = __frcp_rn(ss[v]);
-#pragma unroll
-                for (int a = 0; a < 5; ++a) x[5 * v + a] = (x[5 * v + a] - cc[v]) * r;
-            }
-            mlp_core(W, x, lane, y);
-            if (!need) {
-#pragma unroll
-                for (int o = 0; o < NO; ++o) y[o] = 0.f;
-            }
-#pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                A[v] = stash[0][v][threadIdx.x];
-                G[v] = stash[1][v][threadIdx.x];
-                lo[v] = stash[2][v][threadIdx.x];
-                hi[v] = stash[3][v][threadIdx.x];
-            }
-        } else {
-#pragma unroll
-            for (int o = 0; o < NO; ++o) y[o] = 0.f;
-        }
+        list_append(__ballot(need), lane, t, snet, sc + kCtrNet);
         float lv[8], rv[8];
 #pragma unroll
-        for (int v = 0; v < 8; ++v) {
-            // clamp to [lo, hi] (lo <= hi): the median of the three
-            lv[v] = __builtin_amdgcn_fmed3f(lo[v], hi[v], A[v] + G[v] * y[v]);
-            rv[v] = __builtin_amdgcn_fmed3f(lo[v], hi[v], A[v] + G[v] * y[8 + v]);
-        }
-        const bool fb = !smooth || !(lv[0] > 0.f) || !(rv[0] > 0.f) || !(lv[4] > 0.f) || !(rv[4] > 0.f);
-        // fallback faces go to a list that k_flux_fix evaluates with PDM
-        const unsigned long long fm = __ballot(live && fb);
-        if (fm) {
-            unsigned long long first = 0;
-            const int leader = __ffsll((long long)fm) - 1;
-            if (lane == leader) first = atomicAdd(nfix, (unsigned long long)__popcll(fm));
-            first = __shfl(first, leader, 64);
-            if (live && fb) fix[first + __popcll(fm & ((1ull << lane) - 1ull))] = (int)t;
-        }
+        for (int v = 0; v < 8; ++v) lv[v] = rv[v] = __builtin_amdgcn_fmed3f(lo[v], hi[v], A[v]);
+        const bool fb = live && !need && (!smooth || !sur_positive(lv, rv));
+        const unsigned long long fm = __ballot(fb);
+        list_append(fm, lane, t, sfix, sc + kCtrFix);
         if (stats) {
             const unsigned long long nl = __ballot(live);
             if (lane == 0) {
@@ -622,28 +640,114 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
                 atomicAdd(&stats[1], (unsigned long long)__popcll(fm));
             }
         }
-        if (live && !fb) {
-            if (MET == 2) {
-                float n[3];
-                gen_normal<float, D>(d, gm, i, j, k, n);
-                face_flux_met<float, float>(c, lv, rv, n, F, t, nface);
-            } else if (MET == 1) {
-                face_flux_met<float, float>(c, lv, rv, nm.n[D - 1], F, t, nface);
-            } else {
-                face_flux<float, D>(c, lv, rv, F, t, nface);
-            }
-        }
+        if (live && !need && !fb) sur_face_flux<D, MET>(d, c, nm, gm, i, j, k, lv, rv, F, t, nface);
     }
+}
+
+// pass 2 over one net-list segment of n entries: the workgroup's wavefront
+// w takes entries [64 w, 64 w + 64), then strides by the workgroup's 256
+// (wave-uniform bounds, so every lane of a wave reaches mlp_core together);
+// fallbacks join the same segment's fallback list
+template <int D, int MET>
+__device__ __forceinline__ void flux_net_dir(const NetW& W, float (*stash)[8][kBlock], const Dims& d,
+                                             const Coef<float>& c, const Prim<float>& q, const Normals<float>& nm,
+                                             const GMet& gm, const int* __restrict__ net, long n,
+                                             int* __restrict__ fix, unsigned long long* __restrict__ nfix,
+                                             float* __restrict__ F, long nface,
+                                             unsigned long long* __restrict__ stats) {
+    const int lane = threadIdx.x & 63;
+    const float* f[8] = {q.rho, q.vx, q.vy, q.vz, q.p, q.bx, q.by, q.bz};
+    const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
+    for (long e0 = 64 * (threadIdx.x >> 6); e0 < n; e0 += kBlock) {
+        const long e = e0 + lane;
+        const bool live = e < n;
+        const long t = net[live ? e : e0];  // lanes past the end repeat the wave's first face
+        int i, j, k;
+        unflat(t, ii, jj, i, j, k);
+        float x[NI];
+        face_windows<D>(d, f, i, j, k, x);
+        float cc[8], ss[8], A[8], G[8], lo[8], hi[8];
+        bool smooth, curved;
+        face_parts(x, 0.f, cc, ss, A, G, lo, hi, smooth, curved);
+        // A, G and the clamp range wait in LDS (each thread its own column)
+        // while the MLP holds the registers
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            stash[0][v][threadIdx.x] = A[v];
+            stash[1][v][threadIdx.x] = G[v];
+            stash[2][v][threadIdx.x] = lo[v];
+            stash[3][v][threadIdx.x] = hi[v];
+        }
+        // the net's inputs: windows centred on cc and divided by ss
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            const float r = __frcp_rn(ss[v]);
+#pragma unroll
+            for (int a = 0; a < 5; ++a) x[5 * v + a] = (x[5 * v + a] - cc[v]) * r;
+        }
+        float y[NO];
+        mlp_core(W, x, lane, y);
+        float lv[8], rv[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            const float a = stash[0][v][threadIdx.x], g = stash[1][v][threadIdx.x];
+            const float l = stash[2][v][threadIdx.x], h = stash[3][v][threadIdx.x];
+            // clamp to [lo, hi] (lo <= hi): the median of the three
+            lv[v] = __builtin_amdgcn_fmed3f(l, h, a + g * y[v]);
+            rv[v] = __builtin_amdgcn_fmed3f(l, h, a + g * y[8 + v]);
+        }
+        const bool fb = live && !sur_positive(lv, rv);
+        const unsigned long long fm = __ballot(fb);
+        list_append(fm, lane, t, fix, nfix);
+        if (stats && fm && lane == 0) atomicAdd(&stats[1], (unsigned long long)__popcll(fm));
+        if (live && !fb) sur_face_flux<D, MET>(d, c, nm, gm, i, j, k, lv, rv, F, t, nface);
+    }
+}
+
+// the list segments of the three directions (blockIdx.y = direction,
+// blockIdx.x = segment in the passes over them)
+struct SegLists {
+    int* net[3];                 // segments of each direction, kNetSegCap apart
+    int* fix[3];
+    unsigned long long* ctr[3];  // their counters, kCtrStride apart
+    unsigned nseg[3];
+    float* F[3];
+    long nface[3];
+};
+template <int MET>
+__global__ void __launch_bounds__(kBlock) k_flux_mlp(Dims d, Coef<float> c, Prim<float> q, Normals<float> nm, GMet gm,
+                                                    const unsigned short* __restrict__ wbf,
+                                                    const float* __restrict__ bias, SegLists sl,
+                                                    unsigned long long* __restrict__ stats) {
+    __shared__ NetW W;
+    __shared__ float stash[4][8][kBlock];
+    const int dir = blockIdx.y;
+    const unsigned sg = blockIdx.x;
+    if (sg >= sl.nseg[dir]) return;  // block-uniform
+    unsigned long long* const sc = sl.ctr[dir] + (size_t)sg * kCtrStride;
+    const long n = (long)sc[kCtrNet];
+    if (n == 0) return;  // block-uniform: an empty segment
+    load_net(W, wbf, bias);
+    const size_t o = (size_t)sg * kNetSegCap;
+    if (dir == 0)
+        flux_net_dir<1, MET>(W, stash, d, c, q, nm, gm, sl.net[0] + o, n, sl.fix[0] + o, sc + kCtrFix, sl.F[0],
+                             sl.nface[0], stats);
+    else if (dir == 1)
+        flux_net_dir<2, MET>(W, stash, d, c, q, nm, gm, sl.net[1] + o, n, sl.fix[1] + o, sc + kCtrFix, sl.F[1],
+                             sl.nface[1], stats);
+    else
+        flux_net_dir<3, MET>(W, stash, d, c, q, nm, gm, sl.net[2] + o, n, sl.fix[2] + o, sc + kCtrFix, sl.F[2],
+                             sl.nface[2], stats);
 }
 
 // PDM flux of the listed fallback faces (fp32, as the plain mixed path)
 template <int D, int MET>
 __device__ __forceinline__ void flux_fix_dir(const Dims& d, const Coef<float>& c, const Prim<float>& q,
                                              const Normals<float>& nm, const GMet& gm, const int* __restrict__ fix,
-                                             long n, float* __restrict__ F, long nface, long first, long stride) {
+                                             long n, float* __restrict__ F, long nface) {
     const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
     const ptrdiff_t st = D == 1 ? 1 : (D == 2 ? (ptrdiff_t)d.nx : (ptrdiff_t)d.nx * d.ny);
-    for (long e = first; e < n; e += stride) {
+    for (long e = threadIdx.x; e < n; e += blockDim.x) {
         const long t = fix[e];
         int i, j, k;
         unflat(t, ii, jj, i, j, k);
@@ -669,25 +773,28 @@ __device__ __forceinline__ void flux_fix_dir(const Dims& d, const Coef<float>& c
     }
 }
 
-// the three directions' fallback lists in one launch: blockIdx.y = direction
-struct FixLists {
-    const int* fix[3];
-    float* F[3];
-    long nface[3];
-};
+// the fallback-list segments of the three directions in one launch, one
+// workgroup per segment
 template <int MET>
 __global__ void __launch_bounds__(kBlock) k_flux_fix(Dims d, Coef<float> c, Prim<float> q, Normals<float> nm, GMet gm,
-                                                    FixLists fl, const unsigned long long* __restrict__ nfix) {
-    const long first = gtid(), stride = (long)gridDim.x * blockDim.x;
+                                                    SegLists sl) {
     const int dir = blockIdx.y;
-    const long n = (long)nfix[dir];
-    if (dir == 0) flux_fix_dir<1, MET>(d, c, q, nm, gm, fl.fix[0], n, fl.F[0], fl.nface[0], first, stride);
-    else if (dir == 1) flux_fix_dir<2, MET>(d, c, q, nm, gm, fl.fix[1], n, fl.F[1], fl.nface[1], first, stride);
-    else flux_fix_dir<3, MET>(d, c, q, nm, gm, fl.fix[2], n, fl.F[2], fl.nface[2], first, stride);
+    const unsigned sg = blockIdx.x;
+    if (sg >= sl.nseg[dir]) return;
+    const long n = (long)sl.ctr[dir][(size_t)sg * kCtrStride + kCtrFix];
+    const size_t o = (size_t)sg * kNetSegCap;
+    if (dir == 0) flux_fix_dir<1, MET>(d, c, q, nm, gm, sl.fix[0] + o, n, sl.F[0], sl.nface[0]);
+    else if (dir == 1) flux_fix_dir<2, MET>(d, c, q, nm, gm, sl.fix[1] + o, n, sl.F[1], sl.nface[1]);
+    else flux_fix_dir<3, MET>(d, c, q, nm, gm, sl.fix[2] + o, n, sl.F[2], sl.nface[2]);
 }
 
-__global__ void k_zero_u64(unsigned long long* __restrict__ p, int n) {
-    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+// the segments' counters
+__global__ void k_zero_ctr(unsigned long long* __restrict__ p, long nseg) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nseg) {
+        p[t * kCtrStride + kCtrNet] = 0ull;
+        p[t * kCtrStride + kCtrFix] = 0ull;
+    }
 }
 
 
@@ -727,11 +834,31 @@ void surrogate_free(mhd_ctx* c) {
     SurrogateState* s = c->sur;
     if (!s) return;
     void* ps[] = {s->p, s->g, s->m, s->v, s->samples, s->count, s->scratch, s->loss, s->wbf, s->bias, s->stats,
-                  s->fix, s->nfix};
+                  s->fix, s->net, s->ctr};
     for (void* p : ps)
         if (p) hipFree(p);
     delete s;
     c->sur = nullptr;
+}
+
+// pass-1 workgroups and net-list segments of each direction
+struct SurGrid {
+    unsigned blocks[3], nseg[3];
+    long segs;  // all directions
+};
+static SurGrid sur_grid(const mhd_ctx* c) {
+    const Dims& d = c->d;
+    const unsigned sg = sur::SUR_GROUPS;
+    SurGrid g;
+    g.blocks[0] = (unsigned)((c->nf[0] + sg * kBlock - 1) / (sg * kBlock));
+    g.blocks[1] = (face_tiles<2>(d.nx0, d.ny0 + 1, d.nz0) + sg - 1) / sg;
+    g.blocks[2] = (face_tiles<3>(d.nx0, d.ny0, d.nz0 + 1) + sg - 1) / sg;
+    g.segs = 0;
+    for (int a = 0; a < 3; ++a) {
+        g.nseg[a] = (g.blocks[a] + sur::kNetSeg - 1) / sur::kNetSeg;
+        g.segs += g.nseg[a];
+    }
+    return g;
 }
 
 template <int MET>
@@ -741,26 +868,29 @@ static void flux_sur_launch(mhd_ctx* c, SurrogateState* S, Src<float> s) {
     const Prim<float> q{s.rho, s.vx, s.vy, s.vz, s.p, s.bx, s.by, s.bz};
     const float theta = c->sur_theta;
     unsigned long long* st = c->sur_stats ? S->stats : nullptr;
-    launch(c, MHD_K_SURR, dim3(1), dim3(64), sur::k_zero_u64, S->nfix, 3);
-    int* fx[3] = {S->fix, S->fix + c->nf[0], S->fix + c->nf[0] + c->nf[1]};
-    const Dims& d = c->d;
-    const unsigned sg = sur::SUR_GROUPS;
-    const unsigned gsur[3] = {(unsigned)((c->nf[0] + sg * kBlock - 1) / (sg * kBlock)),
-                              (face_tiles<2>(d.nx0, d.ny0 + 1, d.nz0) + sg - 1) / sg,
-                              (face_tiles<3>(d.nx0, d.ny0, d.nz0 + 1) + sg - 1) / sg};
-    const unsigned short* wbf = S->wbf;
-    const float* bias = S->bias;
-    launch(c, MHD_K_FLUX, dim3(gsur[0]), dim3(kBlock), sur::k_flux_sur<1, MET>, c->d, co, q, nm, c->gm, theta, wbf, bias,
-           (float*)c->flux[0], c->nf[0], fx[0], S->nfix + 0, st);
-    launch(c, MHD_K_FLUX, dim3(gsur[1]), dim3(kBlock), sur::k_flux_sur<2, MET>, c->d, co, q, nm, c->gm, theta, wbf, bias,
-           (float*)c->flux[1], c->nf[1], fx[1], S->nfix + 1, st);
-    launch(c, MHD_K_FLUX, dim3(gsur[2]), dim3(kBlock), sur::k_flux_sur<3, MET>, c->d, co, q, nm, c->gm, theta, wbf, bias,
-           (float*)c->flux[2], c->nf[2], fx[2], S->nfix + 2, st);
-    // the fallback lists: a fixed grid walks whatever length each list has
-    const sur::FixLists fl{{fx[0], fx[1], fx[2]}, {(float*)c->flux[0], (float*)c->flux[1], (float*)c->flux[2]},
-                      {c->nf[0], c->nf[1], c->nf[2]}};
-    launch(c, MHD_K_SURR, dim3(1024, 3), dim3(kBlock), sur::k_flux_fix<MET>, c->d, co, q, nm, c->gm, fl,
-           (const unsigned long long*)S->nfix);
+    const SurGrid sg = sur_grid(c);
+    launch(c, MHD_K_SURR, blocks(sg.segs), dim3(kBlock), sur::k_zero_ctr, S->ctr, sg.segs);
+    sur::SegLists sl{};
+    for (int a = 0, o = 0; a < 3; o += (int)sg.nseg[a], ++a) {
+        sl.net[a] = S->net + (size_t)o * sur::kNetSegCap;
+        sl.fix[a] = S->fix + (size_t)o * sur::kNetSegCap;
+        sl.ctr[a] = S->ctr + (size_t)o * sur::kCtrStride;
+        sl.nseg[a] = sg.nseg[a];
+        sl.F[a] = (float*)c->flux[a];
+        sl.nface[a] = c->nf[a];
+    }
+    launch(c, MHD_K_FLUX, dim3(sg.blocks[0]), dim3(kBlock), sur::k_flux_sur<1, MET>, c->d, co, q, nm, c->gm, theta,
+           sl.F[0], c->nf[0], sl.fix[0], sl.net[0], sl.ctr[0], st);
+    launch(c, MHD_K_FLUX, dim3(sg.blocks[1]), dim3(kBlock), sur::k_flux_sur<2, MET>, c->d, co, q, nm, c->gm, theta,
+           sl.F[1], c->nf[1], sl.fix[1], sl.net[1], sl.ctr[1], st);
+    launch(c, MHD_K_FLUX, dim3(sg.blocks[2]), dim3(kBlock), sur::k_flux_sur<3, MET>, c->d, co, q, nm, c->gm, theta,
+           sl.F[2], c->nf[2], sl.fix[2], sl.net[2], sl.ctr[2], st);
+    // the net lists, then the fallback lists (the net pass appends to them),
+    // one workgroup per segment
+    const unsigned mseg = std::max(sg.nseg[0], std::max(sg.nseg[1], sg.nseg[2]));
+    launch(c, MHD_K_SURR, dim3(mseg, 3), dim3(kBlock), sur::k_flux_mlp<MET>, c->d, co, q, nm, c->gm,
+           (const unsigned short*)S->wbf, (const float*)S->bias, sl, st);
+    launch(c, MHD_K_SURR, dim3(mseg, 3), dim3(kBlock), sur::k_flux_fix<MET>, c->d, co, q, nm, c->gm, sl);
 }
 
 void launch_flux_surrogate(mhd_ctx* c, Src<float> s) {
@@ -769,13 +899,21 @@ void launch_flux_surrogate(mhd_ctx* c, Src<float> s) {
         launch_flux_fp32(c, s);
         return;
     }
-    const long nfx = c->nf[0] + c->nf[1] + c->nf[2];
-    if (S->fix_n < nfx) {
-        if (S->fix) hipFree(S->fix);
-        if (!S->nfix && hipMalloc(&S->nfix, 64) != hipSuccess) S->nfix = nullptr;
-        if (hipMalloc(&S->fix, (size_t)nfx * sizeof(int)) != hipSuccess) S->fix = nullptr;
-        S->fix_n = S->fix && S->nfix ? nfx : 0;
-        if (!S->fix_n) {  // no room for the fallback lists: plain fp32 PDM
+    const long nsegs = sur_grid(c).segs;
+    if (S->seg_n < nsegs) {
+        void* old[] = {S->fix, S->net, S->ctr};
+        for (void* p : old)
+            if (p) hipFree(p);
+        S->fix = S->net = nullptr;
+        S->ctr = nullptr;
+        const size_t lb = (size_t)nsegs * sur::kNetSegCap * sizeof(int);
+        if (hipMalloc(&S->fix, lb) != hipSuccess) S->fix = nullptr;
+        if (hipMalloc(&S->net, lb) != hipSuccess) S->net = nullptr;
+        if (hipMalloc(&S->ctr, (size_t)nsegs * sur::kCtrStride * sizeof(unsigned long long)) != hipSuccess)
+            S->ctr = nullptr;
+        const bool ok = S->fix && S->net && S->ctr;
+        S->seg_n = ok ? nsegs : 0;
+        if (!ok) {  // no room for the face lists: plain fp32 PDM
             launch_flux_fp32(c, s);
             return;
         }
