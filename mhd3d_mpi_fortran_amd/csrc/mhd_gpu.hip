@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "mhd_launch.hpp"
@@ -216,6 +217,19 @@ int mhd_set_geometry(mhd_ctx* c, const double* dx, const double* dy, const doubl
         hipMemcpyAsync(c->dz, dz, c->d.nz * sizeof(double), hipMemcpyHostToDevice, c->s) != hipSuccess)
         return MHD_ERR_HIP;
     c->have_geometry = true;
+    // periodic ghost cell g repeats owned cell g +- n0 (1-based); with equal
+    // spacings too, a ghost cell's CFL step is its owned twin's
+    c->ghost_twins = true;
+    const std::vector<double>* hs[3] = {&c->hdx, &c->hdy, &c->hdz};
+    const int n0[3] = {c->d.nx0, c->d.ny0, c->d.nz0};
+    for (int a = 0; a < 3; ++a) {
+        if (c->p.bc[a] != MHD_BC_PERIODIC) c->ghost_twins = false;
+        const std::vector<double>& h = *hs[a];
+        for (int g = 1; g <= c->d.ng && c->ghost_twins; ++g) {
+            const int hi = c->d.ng + n0[a] + g;  // upper ghost g
+            if (!(h[g - 1] == h[g + n0[a] - 1]) || !(h[hi - 1] == h[hi - n0[a] - 1])) c->ghost_twins = false;
+        }
+    }
     if (c->met.on && upload_metric_spacing(c) != MHD_OK) return MHD_ERR_HIP;
     return hipStreamSynchronize(c->s) == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
@@ -612,6 +626,12 @@ static BMet bmet(mhd_ctx* c, bool w64, bool w32) {
     return m;
 }
 static bool mixed(const mhd_ctx* c) { return c->p.precision == MHD_MIXED; }
+// the mixed Adams-Bashforth update takes the CFL partials of the new state
+// itself (instead of a get_bxyz pass over the block) where every ghost cell
+// repeats an owned cell
+static bool fuse_dt(const mhd_ctx* c) {
+    return mixed(c) && c->p.stepop == 1 && c->met.mode != 2 && c->ghost_twins;
+}
 
 // the 11 state arrays of the current state (AB rotation order)
 static double** state_slot(mhd_ctx* c, int a) {
@@ -663,6 +683,7 @@ static void do_bxyz(mhd_ctx* c, bool want_vf, bool want_shadow, bool want_dt, bo
            store_cellb ? c->bx : (double*)nullptr, c->by, c->bz, want_vf ? c->vf : (double*)nullptr, sh, m,
            sdx(c), sdy(c), sdz(c), want_dt ? c->dtpart : (double*)nullptr, c->ncell, c->nbi, c->nbj, c->nbk);
     c->dt_fresh = want_dt;
+    c->ndt_valid = blocks(c->nmax).x;
     c->cellb_stale = !store_cellb;
 }
 
@@ -676,7 +697,7 @@ static void do_dt(mhd_ctx* c, bool set_tau) {
                c->scal + SC_DTCFL, set_tau ? c->scal + SC_TAU : (double*)nullptr);
         return;
     }
-    launch(c, MHD_K_DT, dim3(1), dim3(1024), k_dt_final, (const double*)c->dtpart, c->ndtpart, c->scal + SC_DTCFL,
+    launch(c, MHD_K_DT, dim3(1), dim3(1024), k_dt_final, (const double*)c->dtpart, c->ndt_valid, c->scal + SC_DTCFL,
            set_tau ? c->scal + SC_TAU : (double*)nullptr);
 }
 
@@ -695,9 +716,26 @@ static void do_cells_t(mhd_ctx* c, double* const* out) {
     PrevRK pv{c->rhop, c->mxp, c->myp, c->mzp, c->engp};
     FluxSet<TF> F{(const TF*)c->flux[0], (const TF*)c->flux[1], (const TF*)c->flux[2], c->nf[0], c->nf[1], c->nf[2]};
     const bool gen = c->met.mode == 2;
+    if constexpr (NST == 0 && std::is_same_v<TF, float>) {
+        if (fuse_dt(c)) {  // the CFL partials of the new state too (CT has run)
+            UpdDt u{};
+            u.bi = out[8];
+            u.bj = out[9];
+            u.bk = out[10];
+            u.on = c->met.on ? 1 : 0;
+            for (int q = 0; q < 9; ++q) u.M[q] = c->met.M[q];
+            u.gam = c->p.gam;
+            u.cfl = c->p.cfl;
+            u.part = c->dtpart;
+            launch(c, MHD_K_UPDATE, blocks(c->ninterior), dim3(kBlock), k_update<0, 0, float, false, true>, c->d,
+                   c->p.gam - 1., (const double*)(c->scal + SC_TAU), s, so, pv, F, sdx(c), sdy(c), sdz(c), gmet(c),
+                   c->ninterior, u);
+            return;
+        }
+    }
     auto ku = gen ? k_update<NST, ST, TF, true> : k_update<NST, ST, TF, false>;
     launch(c, MHD_K_UPDATE, blocks(c->ninterior), dim3(kBlock), ku, c->d, c->p.gam - 1.,
-           (const double*)(c->scal + SC_TAU), s, so, pv, F, sdx(c), sdy(c), sdz(c), gmet(c), c->ninterior);
+           (const double*)(c->scal + SC_TAU), s, so, pv, F, sdx(c), sdy(c), sdz(c), gmet(c), c->ninterior, UpdDt{});
 }
 
 // CT face update from the stage's EMFs into `out`
@@ -737,13 +775,16 @@ static void stage_overlapped(mhd_ctx* c, E emf, F flux, double* const* out) {
         if (mixed(c)) do_ct_t<float, NST, ST>(c, out);
         else do_ct_t<double, NST, ST>(c, out);
     };
+    // an update that takes the CFL partials of the new state reads CT's faces
+    const bool after_ct = NST == 0 && fuse_dt(c);
     const bool ok = c->overlap && hipEventRecord(c->ev_fork, c->s) == hipSuccess &&
                     hipStreamWaitEvent(c->s2, c->ev_fork, 0) == hipSuccess;
     if (!ok) {   // serial
         emf();
         flux();
+        if (after_ct) ct();
         cells();
-        ct();
+        if (!after_ct) ct();
         return;
     }
     c->cur = c->s2;
@@ -753,9 +794,9 @@ static void stage_overlapped(mhd_ctx* c, E emf, F flux, double* const* out) {
     hipEventRecord(c->ev_join, c->s2);
     c->cur = nullptr;
     flux();
-    hipStreamWaitEvent(c->s, c->ev_emf, 0);
+    hipStreamWaitEvent(c->s, after_ct ? c->ev_join : c->ev_emf, 0);
     cells();
-    hipStreamWaitEvent(c->s, c->ev_join, 0);
+    if (!after_ct) hipStreamWaitEvent(c->s, c->ev_join, 0);
 }
 
 template <int NST, int ST>
@@ -934,7 +975,13 @@ static void step_ab(mhd_ctx* c) {
         c->xp[a] = cur;
     }
     do_fill(c);
-    end_of_stage(c, true);
+    if (fuse_dt(c)) {  // the update left the CFL partials of the new state
+        c->dt_fresh = true;
+        c->ndt_valid = blocks(c->ninterior).x;
+        c->cellb_stale = true;
+    } else {
+        end_of_stage(c, true);
+    }
 }
 
 int mhd_step(mhd_ctx* c, int nsteps, double* tau, double* tau_cfl) {

@@ -82,6 +82,8 @@ struct mhd_ctx {
     // fp32 stencil inputs (mixed precision): rho vx vy vz p bx by bz vf bi bj bk
     float* sh[12] = {};
     bool dt_fresh = false;   // dtpart holds the CFL partials of the current state
+    long ndt_valid = 0;      // ... that many of them
+    bool ghost_twins = false;  // every ghost cell repeats an owned cell (periodic axes, equal ghost spacings)
     bool cellb_stale = false;  // bx,by,bz not stored for the current state (mixed AB steps)
     // per-stage products (binary64, or fp32 in mixed precision)
     void* flux[3] = {nullptr, nullptr, nullptr};

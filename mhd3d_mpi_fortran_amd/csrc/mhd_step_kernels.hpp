@@ -30,17 +30,72 @@ struct FluxSet {
     long nfx, nfy, nfz;
 };
 
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ double block_min(double v) {
+    __shared__ double red[kBlock / 64];
+    v = wave_min(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = red[q] < m ? red[q] : m;
+        red[0] = m;
+    }
+    __syncthreads();
+    return red[0];
+}
+
+__device__ __forceinline__ double cfl_dt(double gam, double cfl, double u, double v, double w, double x, double y,
+                                         double z, double r, double pp, double dxi, double dyj, double dzk) {
+    const double vv = sqrt(u * u + v * v + w * w);
+    const double btot = sqrt(x * x + y * y + z * z);
+    const double valf = btot / sqrt(r);
+    const double vs = sqrt(gam * pp / r);
+    const double vcfl = vv + sqrt(valf * valf + vs * vs);
+    return cfl / (vcfl / dxi + vcfl / dyj + vcfl / dzk);
+}
+
+// CFL partials of the updated state, computed by the update itself (get_dt
+// on the new state, mhd3d3.f90:1608-1639): cell B from the stage's new
+// faces (CT has written them before the update runs; the affine metric's
+// cell-B map when on), the CFL step of every owned cell, one minimum per
+// workgroup.  Owned cells only: the caller takes this where every ghost
+// cell repeats an owned cell (periodic axes whose ghost spacings repeat the
+// owned ones), so the minimum is get_dt's over the ghost-inclusive block.
+struct UpdDt {
+    const double *bi, *bj, *bk;  // the new faces
+    double M[9];                 // affine metric's cell-B map (on != 0)
+    int on;
+    double gam, cfl;
+    double* part;                // one partial per workgroup
+};
+
 // GEN: general metric (gm), div = (sum_f F_f |S_f| out) / V; a template
-// parameter so that the other grids' kernel carries none of its registers
-template <int NST, int ST, typename TF, bool GEN>
+// parameter so that the other grids' kernel carries none of its registers.
+// DT: CFL partials of the new state as well (UpdDt)
+template <int NST, int ST, typename TF, bool GEN, bool DT = false>
 __global__ void __launch_bounds__(kBlock) k_update(Dims d, double gm1, const double* __restrict__ tau_p, CCells s,
                                                   Cells so, PrevRK pv, FluxSet<TF> F, const double* __restrict__ dx,
                                                   const double* __restrict__ dy, const double* __restrict__ dz,
-                                                  GMet gm, long ncell) {
+                                                  GMet gm, long ncell, UpdDt udt) {
     const long t = gtid_xcd();
-    if (t >= ncell) return;
+    if constexpr (!DT) {
+        if (t >= ncell) return;
+    }
+    // with DT every thread reaches the workgroup minimum: threads past the
+    // end work on cell 1 and store nothing
+    const bool live = t < ncell;
     int i, j, k;
-    unflat(t, d.nx0, d.ny0, i, j, k);
+    unflat(live ? t : 0, d.nx0, d.ny0, i, j, k);
     const int ng = d.ng;
     const ptrdiff_t q = ix(d.nx, d.ny, ng + i, ng + j, ng + k);
     const double tau = *tau_p;
@@ -115,11 +170,33 @@ __global__ void __launch_bounds__(kBlock) k_update(Dims d, double gm1, const dou
         my = (pv.my[q] + 2. * my) / 3.;
         mz = (pv.mz[q] + 2. * mz) / 3.;
     }
-    so.rho[q] = rho;
-    so.p[q] = p;
-    so.vx[q] = mx / rho;
-    so.vy[q] = my / rho;
-    so.vz[q] = mz / rho;
+    const double un = mx / rho, vn = my / rho, wn = mz / rho;
+    if (live) {
+        so.rho[q] = rho;
+        so.p[q] = p;
+        so.vx[q] = un;
+        so.vy[q] = vn;
+        so.vz[q] = wn;
+    }
+    if constexpr (DT) {
+        double dtc = __builtin_huge_val();
+        if (live) {
+            // get_bxyz of the new faces, as k_bxyz
+            const ptrdiff_t gi = ix(d.nx + 1, d.ny, ng + i, ng + j, ng + k), gj = ix(d.nx, d.ny + 1, ng + i, ng + j, ng + k);
+            double x = .5 * (udt.bi[gi + 1] + udt.bi[gi]);
+            double y = .5 * (udt.bj[gj + d.nx] + udt.bj[gj]);
+            double z = .5 * (udt.bk[q + (ptrdiff_t)d.nx * d.ny] + udt.bk[q]);
+            if (udt.on) {
+                const double fx = x, fy = y, fz = z;
+                x = udt.M[0] * fx + udt.M[1] * fy + udt.M[2] * fz;
+                y = udt.M[3] * fx + udt.M[4] * fy + udt.M[5] * fz;
+                z = udt.M[6] * fx + udt.M[7] * fy + udt.M[8] * fz;
+            }
+            dtc = cfl_dt(udt.gam, udt.cfl, un, vn, wn, x, y, z, rho, p, dxi, dyj, dzk);
+        }
+        dtc = block_min(dtc);
+        if (threadIdx.x == 0) udt.part[blockIdx.x] = dtc;
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -270,30 +347,6 @@ __global__ void __launch_bounds__(kBlock) k_fill(FillSet fs, int ng) {
     A.a[ix(len[0], len[1], i, j, k)] = A.a[ix(len[0], len[1], si, sj, sk)];
 }
 
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const double w = __shfl_xor(v, o, 64);
-        v = w < v ? w : v;
-    }
-    return v;
-}
-
-__device__ __forceinline__ double block_min(double v) {
-    __shared__ double red[kBlock / 64];
-    v = wave_min(v);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) red[wid] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = red[0];
-        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = red[q] < m ? red[q] : m;
-        red[0] = m;
-    }
-    __syncthreads();
-    return red[0];
-}
-
 // ------------------------------------------------------------------------
 // get_bxyz (mhd3d3.f90:2118-2128) over the ghost-inclusive block, fused with
 //   * the EMF's fast speed vf=|v|+|B|/sqrt(rho) (mhd3d3.f90:1766) when vf != 0,
@@ -337,16 +390,6 @@ __device__ __forceinline__ double cfl_dt_gen(double gam, double cfl, double u, d
     const double vs = sqrt(gam * pp / r);
     const double vcfl = vv + sqrt(valf * valf + vs * vs);
     return cfl / (vcfl * g.ih[t] + vcfl * g.ih[g.ncell + t] + vcfl * g.ih[2 * g.ncell + t]);
-}
-
-__device__ __forceinline__ double cfl_dt(double gam, double cfl, double u, double v, double w, double x, double y,
-                                         double z, double r, double pp, double dxi, double dyj, double dzk) {
-    const double vv = sqrt(u * u + v * v + w * w);
-    const double btot = sqrt(x * x + y * y + z * z);
-    const double valf = btot / sqrt(r);
-    const double vs = sqrt(gam * pp / r);
-    const double vcfl = vv + sqrt(valf * valf + vs * vs);
-    return cfl / (vcfl / dxi + vcfl / dyj + vcfl / dzk);
 }
 
 // GEN: general metric (met.g)
