@@ -523,6 +523,41 @@ __device__ __forceinline__ void face_windows(const Dims& d, const float* const* 
     }
 }
 
+// pass 1's windows in a strided sweep (D = 2, 3), through LDS: the face
+// tile's 64 x-columns by 8 rows along the sweep (its 4 face planes and their
+// taps) are loaded once, each row coalesced, and every face reads its 5 taps
+// per variable from there; the wavefronts would otherwise fetch each row
+// 5 times over.  Every thread of the workgroup calls it (two barriers);
+// columns and rows past the block's edges are clamped onto it (their faces
+// are not stored).
+template <int D>
+__device__ __forceinline__ void tile_windows(const Dims& d, const float* const* f, unsigned tile, int ii, int jj,
+                                             int kk, float* x) {
+    __shared__ float win[8][kTileN + 4][kTileX];
+    const unsigned nn = D == 2 ? jj : kk, no = D == 2 ? kk : jj;
+    const unsigned ntx = (ii + kTileX - 1) / kTileX, ntn = (nn + kTileN - 1) / kTileN;
+    const unsigned tx = tile % ntx, r = tile / ntx, tn = r % ntn, o = r / ntn;
+    const int lane = threadIdx.x & (kTileX - 1), q = threadIdx.x / kTileX;
+    const int cx = d.ng + 1 + min((int)(tx * kTileX) + lane, ii - 1);
+    const int co = d.ng + 1 + min((int)o, (int)no - 1);
+    const int cmax = D == 2 ? d.ny : d.nz;
+    __syncthreads();  // the previous group's reads are done
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        // row w + a holds tap a of face plane w: cell ng + 4 tn - 1 + row
+        const int row = q + kTileN * h;
+        const int cs = min(d.ng + (int)(tn * kTileN) - 1 + row, cmax);
+        const unsigned at = (unsigned)(D == 2 ? ix(d.nx, d.ny, cx, cs, co) : ix(d.nx, d.ny, cx, co, cs));
+#pragma unroll
+        for (int v = 0; v < 8; ++v) win[v][row][lane] = f[v][at];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 8; ++v)
+#pragma unroll
+        for (int a = 0; a < 5; ++a) x[5 * v + a] = win[v][q + a][lane];
+}
+
 // per variable, in physical units: the window centre cc and normalisation
 // scale ss, the linear face value A = lin of the window (the 4th-order
 // interpolation commutes with the affine normalisation: its weights sum to
@@ -612,13 +647,15 @@ __global__ void __launch_bounds__(kBlock) k_flux_sur(Dims d, Coef<float> c, Prim
             if (__ballot(live) == 0ull) break;  // wave-uniform: later groups are past the end too
             unflat(live ? t : 0, ii, jj, i, j, k);
         } else {
+            // every wavefront stays (tile_windows holds barriers), also one
+            // whose plane lies past the sweep's end
             live = tile_face<D>(g, ii, jj, kk, i, j, k);
-            if (__ballot(live) == 0ull) continue;  // a tile's last planes can lie past the sweep's end
             if (!live) i = j = k = 1;
             t = ix(ii, jj, i, j, k);
         }
         float x[NI];
-        face_windows<D>(d, f, i, j, k, x);
+        if constexpr (D == 1) face_windows<D>(d, f, i, j, k, x);
+        else tile_windows<D>(d, f, g, ii, jj, kk, x);
         float cc[8], ss[8], A[8], G[8], lo[8], hi[8];
         bool smooth, curved;
         face_parts(x, theta, cc, ss, A, G, lo, hi, smooth, curved);
