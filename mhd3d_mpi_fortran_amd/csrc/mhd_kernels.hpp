@@ -247,28 +247,61 @@ __device__ __forceinline__ void gen_normal(const Dims& d, const GMet& g, int i, 
     n[2] = R(a[2 * nb + f]);
 }
 
+// fp32 strided sweeps (D = 2, 3): the face tile's 64 x-columns by
+// kTileN + n - 1 rows along the sweep (its 4 face planes and their n taps)
+// are loaded into LDS once, each row coalesced, and every face reconstructs
+// from there; the wavefronts would otherwise fetch each row n times over.
+// Every thread of the workgroup calls it (a barrier); columns and rows past
+// the block's edges are clamped onto it (their faces are not stored).
+template <typename R, int M, int D, typename TI>
+__device__ __forceinline__ void tile_recon(const Dims& d, const Prim<TI>& q, const Coef<R>& c, unsigned tile, int ii,
+                                           int jj, int kk, R* l, R* r) {
+    using S = Stencil<R, M>;
+    constexpr int NR = kTileN + S::n - 1;
+    __shared__ TI win[8][NR][kTileX];
+    const TI* f[8] = {q.rho, q.vx, q.vy, q.vz, q.p, q.bx, q.by, q.bz};
+    const unsigned nn = D == 2 ? jj : kk, no = D == 2 ? kk : jj;
+    const unsigned ntx = (ii + kTileX - 1) / kTileX, ntn = (nn + kTileN - 1) / kTileN;
+    const unsigned tx = tile % ntx, rr = tile / ntx, tn = rr % ntn, o = rr / ntn;
+    const int lane = threadIdx.x & (kTileX - 1), w = threadIdx.x / kTileX;
+    const int cx = d.ng + 1 + min((int)(tx * kTileX) + lane, ii - 1);
+    const int co = d.ng + 1 + min((int)o, (int)no - 1);
+    const int cmax = D == 2 ? d.ny : d.nz;
+#pragma unroll
+    for (int row = w; row < NR; row += kTileN) {
+        // row w + m holds tap m of face plane w: cell ng + 4 tn + 1 + lo + row
+        const int cs = max(1, min(d.ng + (int)(tn * kTileN) + 1 + S::lo + row, cmax));
+        const unsigned at = (unsigned)(D == 2 ? ix(d.nx, d.ny, cx, cs, co) : ix(d.nx, d.ny, cx, co, cs));
+#pragma unroll
+        for (int v = 0; v < 8; ++v) win[v][row][lane] = f[v][at];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        R t[S::n];
+#pragma unroll
+        for (int m = 0; m < S::n; ++m) t[m] = R(win[v][w + m][lane]);
+        S::run(t, c, l[v], r[v]);
+    }
+}
+
 // MET = 0: the reference's axis-aligned flux; 1: along the affine metric's
 // normals; 2: along the general metric's per-face normals
 template <typename R, int M, int D, typename TI, typename TF, int MET>
 __global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<TI> q, Normals<R> nm, GMet gm,
                                                  TF* __restrict__ F, long nface) {
     const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2), kk = d.nz0 + (D == 3);
+    const ptrdiff_t st = D == 1 ? 1 : (D == 2 ? (ptrdiff_t)d.nx : (ptrdiff_t)d.nx * d.ny);
     int i, j, k;
     long t;
-    if constexpr (D == 1) {  // flat: the x sweep's taps are contiguous
+    R l[8], r[8];
+    if constexpr (D == 1) {
+        // flat; unit stride: the taps of one array are contiguous and load
+        // as wide vectors from one address
         t = gtid_xcd();
         if (t >= nface) return;
         unflat(t, ii, jj, i, j, k);
-    } else {
-        if (!tile_face<D>(xcd_block(), ii, jj, kk, i, j, k)) return;
-        t = ix(ii, jj, i, j, k);
-    }
-    const ptrdiff_t base = ix(d.nx, d.ny, d.ng + i, d.ng + j, d.ng + k);
-    const ptrdiff_t st = D == 1 ? 1 : (D == 2 ? (ptrdiff_t)d.nx : (ptrdiff_t)d.nx * d.ny);
-    R l[8], r[8];
-    if (D == 1) {
-        // unit stride: the taps of one array are contiguous and load as
-        // wide vectors from one address
+        const ptrdiff_t base = ix(d.nx, d.ny, d.ng + i, d.ng + j, d.ng + k);
         recon_line<R, M>(q.rho, base, st, c, l[0], r[0]);
         recon_line<R, M>(q.vx, base, st, c, l[1], r[1]);
         recon_line<R, M>(q.vy, base, st, c, l[2], r[2]);
@@ -277,8 +310,18 @@ __global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<TI> q, 
         recon_line<R, M>(q.bx, base, st, c, l[5], r[5]);
         recon_line<R, M>(q.by, base, st, c, l[6], r[6]);
         recon_line<R, M>(q.bz, base, st, c, l[7], r[7]);
+    } else if constexpr (sizeof(TI) == 4) {
+        // fp32 stencil inputs: the tile's rows through LDS
+        const unsigned tile = xcd_block();
+        const bool live = tile_face<D>(tile, ii, jj, kk, i, j, k);
+        tile_recon<R, M, D, TI>(d, q, c, tile, ii, jj, kk, l, r);
+        if (!live) return;
+        t = ix(ii, jj, i, j, k);
     } else {
-        const Taps<R, M, TI> tp(base, st);
+        // binary64 (VALU-bound): taps from the tile's rows in L1
+        if (!tile_face<D>(xcd_block(), ii, jj, kk, i, j, k)) return;
+        t = ix(ii, jj, i, j, k);
+        const Taps<R, M, TI> tp(ix(d.nx, d.ny, d.ng + i, d.ng + j, d.ng + k), st);
         recon_taps<R, M>(q.rho, tp, c, l[0], r[0]);
         recon_taps<R, M>(q.vx, tp, c, l[1], r[1]);
         recon_taps<R, M>(q.vy, tp, c, l[2], r[2]);

@@ -317,32 +317,52 @@ def test_solver_with_learned_reconstruction(trained):
 @pytest.mark.parametrize("grid", ["cartesian", "sheared", "shell"])
 def test_all_fallback_equals_mixed_pdm(grid):
     """theta < 0 sends every face to the PDM fallback list: the learned path
-    (flux kernel, one-launch fix-up of the three lists) must then reproduce
-    the mixed-precision PDM path bit for bit, on the reference's grid, the
-    affine sheared grid and the spherical shell's node grid"""
+    (flux kernel, fix-up of the fallback lists) must then reproduce the
+    mixed-precision PDM path, on the reference's grid, the affine sheared grid
+    and the spherical shell's node grid.  The two are different kernels (the
+    plain flux reconstructs the strided sweeps from rows staged in LDS, the
+    fix-up walks its lists through global memory), and fp32 contraction may
+    round them differently in the last place (and a component that cancels,
+    such as the z energy flux of the Orszag-Tang vortex, keeps that rounding
+    at the scale of its terms), so the fluxes of one state must agree to 1e-5
+    of the largest flux of their direction and a 6-step trajectory to 1e-5; a
+    face the fix-up missed or misplaced would differ at O(1)."""
     from mhd3d_mpi_fortran_amd import shell
     if grid == "shell":
         g = shell.spherical_shell(16, 16, 24, ng=4, theta0=.6)
-        st = shell.dipole_inflow(g, 9)
+        states = [shell.dipole_inflow(g, 9)]
         shp, bc = (16, 16, 24), ("even", "even", "periodic")
     else:
         A = [[1., .5, 0.], [0., 1., .2], [0., 0., 1.]] if grid == "sheared" else None
         g = ics.Grid(20, 16, 12, ng=4, A=A)
-        st = ics.initial_state(g, "blast", 13, radius=.2)
+        states = [ics.initial_state(g, "blast", 13, radius=.2)]
+        if A is None:   # (the Orszag-Tang vortex is set up on the reference's grid only)
+            states.append(ics.initial_state(g, "orszag_tang", 14))
         shp, bc = (20, 16, 12), "periodic"
-    out = []
-    for sur in (0, 1):
-        s = gpu.Solver(gpu.make_params(*shp, ng=4, reconop=1, stepop=1, bc=bc, precision="mixed", surrogate=sur), g)
-        if sur:
-            sg.set_weights(s, _weights(11, 1.))
-            sg.configure(s, -1., count=True)
-        s.upload(st)
-        tau = s.initialize(1e-4)
-        s.step(6, tau)
-        out.append(s.download())
-        if sur:
-            faces, fb = sg.stats(s)
-            assert faces > 0 and fb == faces
-        s.close()
-    for k in out[0]:
-        assert np.array_equal(out[0][k], out[1][k]), k
+    for n_state, st in enumerate(states):
+        flux, out = [], []
+        for sur in (0, 1):
+            s = gpu.Solver(gpu.make_params(*shp, ng=4, reconop=1, stepop=1, bc=bc, precision="mixed", surrogate=sur),
+                           g)
+            if sur:
+                sg.set_weights(s, _weights(11, 1.))
+                sg.configure(s, -1., count=True)
+            s.upload(st)
+            s.fill()
+            flux.append(s.flux_stress())
+            if n_state == 0:
+                s.upload(st)
+                tau = s.initialize(1e-4)
+                s.step(6, tau)
+                out.append(s.download())
+            if sur:
+                faces, fb = sg.stats(s)
+                assert faces > 0 and fb == faces
+            s.close()
+        for q in range(24):
+            scale = max(float(np.abs(flux[0][8 * (q // 8) + c]).max()) for c in range(8))
+            assert np.isfinite(flux[1][q]).all()
+            assert np.abs(flux[0][q] - flux[1][q]).max() <= 1e-5 * scale, (n_state, q)
+        for k in (out[0] if out else ()):
+            scale = np.abs(out[0][k]).max() + 1e-300
+            assert np.abs(out[0][k] - out[1][k]).max() <= 1e-5 * scale, k
