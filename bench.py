@@ -56,7 +56,6 @@ ALG_BYTES = {"emf": {"fp64": 5 * 8 + 8, "mixed": 5 * 4 + 4},
 # compulsory state traffic of one Adams-Bashforth step per zone: read U^n and
 # U^(n-1), write U^(n+1), 8 binary64 words each (rho, rho v, p, three faces)
 AB2_STATE_BYTES = 3 * 8 * 8
-BF16_PEAK_TFLOPS = 2500.0
 MLP_FLOPS = 2 * (40 * 32 + 32 * 32 + 32 * 16)
 
 
@@ -478,13 +477,11 @@ def main():
                             "frac": value * AB2_STATE_BYTES / 1e9 / HBM_PEAK_GBS}
     if sur and head_name == "fast":
         s = head["sur"]
-        faces_per_step = sum(head["units"]["flux"])
-        fl_ms = head["flux_timed"][1] / K
+        # the MLP runs only on the listed faces that use the net (k_flux_mlp,
+        # in the "surrogate" kernel class with the fallback fix-up)
         out["surrogate"] = {"theta": sg.THETA, "fallback_fraction": s["fallbacks"] / max(s["faces"], 1),
                             "faces_counted": s["faces"], "training": train_info,
-                            "mlp_flops_per_face": MLP_FLOPS,
-                            "mlp_tflops_in_flux_kernels": faces_per_step * MLP_FLOPS / (fl_ms * 1e-3) / 1e12,
-                            "bf16_peak_tflops": BF16_PEAK_TFLOPS}
+                            "mlp_flops_per_net_face": MLP_FLOPS}
     if args.shell_steps > 0:
         log("spherical-shell record (%d steps) ..." % args.shell_steps)
         out["shell_config"] = shell_config(args.shell_steps)
