@@ -523,39 +523,39 @@ __device__ __forceinline__ void face_windows(const Dims& d, const float* const* 
     }
 }
 
-// pass 1's windows in a strided sweep (D = 2, 3), through LDS: the face
-// tile's 64 x-columns by 8 rows along the sweep (its 4 face planes and their
-// taps) are loaded once, each row coalesced, and every face reads its 5 taps
-// per variable from there; the wavefronts would otherwise fetch each row
-// 5 times over.  Every thread of the workgroup calls it (two barriers);
-// columns and rows past the block's edges are clamped onto it (their faces
-// are not stored).
+// pass 1's windows in a strided sweep (D = 2, 3), through LDS.  A workgroup
+// marches along the sweep over SUR_GROUPS face tiles of one x-column run:
+// tile tn holds face planes 4 tn .. 4 tn + 3 and reads sweep rows
+// 4 tn .. 4 tn + 7 (row rho = cell ng - 1 + rho: the planes' 5 taps), so
+// consecutive tiles share 4 rows.  The rows sit in an LDS ring of 8 (row rho
+// in slot rho mod 8), 64 x-columns per variable; the first tile loads all 8
+// rows, every later one only its 4 new rows, each row one coalesced load,
+// and every face reads its 5 taps per variable from the ring.  Every thread
+// of the workgroup calls it, tile after tile (two barriers); columns and rows
+// past the block's edges are clamped onto it (their faces are not stored).
 template <int D>
-__device__ __forceinline__ void tile_windows(const Dims& d, const float* const* f, unsigned tile, int ii, int jj,
-                                             int kk, float* x) {
-    __shared__ float win[8][kTileN + 4][kTileX];
-    const unsigned nn = D == 2 ? jj : kk, no = D == 2 ? kk : jj;
-    const unsigned ntx = (ii + kTileX - 1) / kTileX, ntn = (nn + kTileN - 1) / kTileN;
-    const unsigned tx = tile % ntx, r = tile / ntx, tn = r % ntn, o = r / ntn;
+__device__ __forceinline__ void ring_windows(const Dims& d, const float* const* f, unsigned tx, unsigned tn,
+                                             unsigned o, int ii, unsigned no, bool first, float* x) {
+    __shared__ float win[8][8][kTileX];
     const int lane = threadIdx.x & (kTileX - 1), q = threadIdx.x / kTileX;
     const int cx = d.ng + 1 + min((int)(tx * kTileX) + lane, ii - 1);
     const int co = d.ng + 1 + min((int)o, (int)no - 1);
     const int cmax = D == 2 ? d.ny : d.nz;
-    __syncthreads();  // the previous group's reads are done
+    __syncthreads();  // the previous tile's reads are done
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        // row w + a holds tap a of face plane w: cell ng + 4 tn - 1 + row
-        const int row = q + kTileN * h;
-        const int cs = min(d.ng + (int)(tn * kTileN) - 1 + row, cmax);
+        if (h == 0 && !first) continue;  // rows 4 tn .. 4 tn + 3 are in the ring already
+        const int row = 4 * (int)tn + q + kTileN * h;
+        const int cs = min(d.ng - 1 + row, cmax);
         const unsigned at = (unsigned)(D == 2 ? ix(d.nx, d.ny, cx, cs, co) : ix(d.nx, d.ny, cx, co, cs));
 #pragma unroll
-        for (int v = 0; v < 8; ++v) win[v][row][lane] = f[v][at];
+        for (int v = 0; v < 8; ++v) win[v][row & 7][lane] = f[v][at];
     }
     __syncthreads();
 #pragma unroll
     for (int v = 0; v < 8; ++v)
 #pragma unroll
-        for (int a = 0; a < 5; ++a) x[5 * v + a] = win[v][q + a][lane];
+        for (int a = 0; a < 5; ++a) x[5 * v + a] = win[v][(4 * (int)tn + q + a) & 7][lane];
 }
 
 // per variable, in physical units: the window centre cc and normalisation
@@ -629,15 +629,23 @@ __global__ void __launch_bounds__(kBlock) k_flux_sur(Dims d, Coef<float> c, Prim
     const float* f[8] = {q.rho, q.vx, q.vy, q.vz, q.p, q.bx, q.by, q.bz};
     const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2), kk = d.nz0 + (D == 3);
     const unsigned lb = xcd_block<kXcdRun / SUR_GROUPS>();
+    // strided sweeps: this workgroup's x-column run tx, tile run tnc along the
+    // sweep (tiles SUR_GROUPS tnc + grp) and transverse index o
+    const unsigned nn = D == 2 ? jj : kk, no = D == 2 ? kk : jj;
+    const unsigned ntx = (ii + kTileX - 1) / kTileX, ntn = (nn + kTileN - 1) / kTileN;
+    const unsigned ntc = (ntn + SUR_GROUPS - 1) / SUR_GROUPS;
+    const unsigned tx = lb % ntx, tnc = lb / ntx % ntc, o = lb / ntx / ntc;
     // this workgroup's list segments and their counters
     int* const sfix = fix + (size_t)(lb / kNetSeg) * kNetSegCap;
     int* const snet = net + (size_t)(lb / kNetSeg) * kNetSegCap;
     unsigned long long* const sc = ctr + (size_t)(lb / kNetSeg) * kCtrStride;
     for (int grp = 0; grp < SUR_GROUPS; ++grp) {
-        // groups of 256 faces: flat for the x sweep, face tiles for the
-        // strided sweeps.  Lanes past the end read the first face's windows
-        // (their results are never written), so the loads need no branches.
+        // groups of 256 faces: flat for the x sweep, face tiles marching along
+        // the strided sweeps.  Lanes past the end read the first face's
+        // windows (their results are never written), so the loads need no
+        // branches.
         const unsigned g = lb * SUR_GROUPS + grp;
+        const unsigned tn = tnc * SUR_GROUPS + grp;
         int i, j, k;
         long t;
         bool live;
@@ -647,15 +655,15 @@ __global__ void __launch_bounds__(kBlock) k_flux_sur(Dims d, Coef<float> c, Prim
             if (__ballot(live) == 0ull) break;  // wave-uniform: later groups are past the end too
             unflat(live ? t : 0, ii, jj, i, j, k);
         } else {
-            // every wavefront stays (tile_windows holds barriers), also one
+            // every wavefront stays (ring_windows holds barriers), also one
             // whose plane lies past the sweep's end
-            live = tile_face<D>(g, ii, jj, kk, i, j, k);
+            live = tn < ntn && tile_face<D>(tx + ntx * (tn + ntn * o), ii, jj, kk, i, j, k);
             if (!live) i = j = k = 1;
             t = ix(ii, jj, i, j, k);
         }
         float x[NI];
         if constexpr (D == 1) face_windows<D>(d, f, i, j, k, x);
-        else tile_windows<D>(d, f, g, ii, jj, kk, x);
+        else ring_windows<D>(d, f, tx, tn, o, ii, no, grp == 0, x);
         float cc[8], ss[8], A[8], G[8], lo[8], hi[8];
         bool smooth, curved;
         face_parts(x, theta, cc, ss, A, G, lo, hi, smooth, curved);
@@ -888,8 +896,10 @@ static SurGrid sur_grid(const mhd_ctx* c) {
     const unsigned sg = sur::SUR_GROUPS;
     SurGrid g;
     g.blocks[0] = (unsigned)((c->nf[0] + sg * kBlock - 1) / (sg * kBlock));
-    g.blocks[1] = (face_tiles<2>(d.nx0, d.ny0 + 1, d.nz0) + sg - 1) / sg;
-    g.blocks[2] = (face_tiles<3>(d.nx0, d.ny0, d.nz0 + 1) + sg - 1) / sg;
+    // strided sweeps: x-column runs x tile runs along the sweep x transverse index
+    const unsigned ntx = (d.nx0 + kTileX - 1) / kTileX;
+    g.blocks[1] = ntx * ((d.ny0 + 1 + kTileN * sg - 1) / (kTileN * sg)) * d.nz0;
+    g.blocks[2] = ntx * ((d.nz0 + 1 + kTileN * sg - 1) / (kTileN * sg)) * d.ny0;
     g.segs = 0;
     for (int a = 0; a < 3; ++a) {
         g.nseg[a] = (g.blocks[a] + sur::kNetSeg - 1) / sur::kNetSeg;
