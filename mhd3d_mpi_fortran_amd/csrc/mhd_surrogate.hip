@@ -498,8 +498,10 @@ __global__ void __launch_bounds__(kBlock) k_infer(const float* __restrict__ x, l
 // ---------------------------------------------------------------------------
 constexpr int SUR_GROUPS = 4;
 constexpr int kNetSeg = 16, kNetSegCap = kNetSeg * SUR_GROUPS * kBlock;
-// counters of segment s: [kCtrStride s + kCtrNet] net list, [.. + kCtrFix] fallback list
-constexpr int kCtrStride = 32, kCtrNet = 0, kCtrFix = 16;
+// counters of segment s, each on its own 128-byte line: [kCtrStride s + kCtrNet] net list,
+// [.. + kCtrFix] fallback list, and (when counting) [.. + kCtrFaces] faces evaluated and
+// [.. + kCtrFb] fallbacks, summed into the context's totals at the end of the flux
+constexpr int kCtrStride = 64, kCtrNet = 0, kCtrFix = 16, kCtrFaces = 32, kCtrFb = 48;
 
 // the 8 windows (5 taps each, physical units) of face (i, j, k) along D
 template <int D>
@@ -678,11 +680,11 @@ __global__ void __launch_bounds__(kBlock) k_flux_sur(Dims d, Coef<float> c, Prim
         const bool fb = live && !need && (!smooth || !sur_positive(lv, rv));
         const unsigned long long fm = __ballot(fb);
         list_append(fm, lane, t, sfix, sc + kCtrFix);
-        if (stats) {
+        if (stats) {  // counting: per-segment tallies (one address for all would serialise)
             const unsigned long long nl = __ballot(live);
             if (lane == 0) {
-                atomicAdd(&stats[0], (unsigned long long)__popcll(nl));
-                atomicAdd(&stats[1], (unsigned long long)__popcll(fm));
+                atomicAdd(sc + kCtrFaces, (unsigned long long)__popcll(nl));
+                if (fm) atomicAdd(sc + kCtrFb, (unsigned long long)__popcll(fm));
             }
         }
         if (live && !need && !fb) sur_face_flux<D, MET>(d, c, nm, gm, i, j, k, lv, rv, F, t, nface);
@@ -699,7 +701,7 @@ __device__ __forceinline__ void flux_net_dir(const NetW& W, float (*stash)[8][kB
                                              const GMet& gm, const int* __restrict__ net, long n,
                                              int* __restrict__ fix, unsigned long long* __restrict__ nfix,
                                              float* __restrict__ F, long nface,
-                                             unsigned long long* __restrict__ stats) {
+                                             unsigned long long* __restrict__ nfb) {
     const int lane = threadIdx.x & 63;
     const float* f[8] = {q.rho, q.vx, q.vy, q.vz, q.p, q.bx, q.by, q.bz};
     const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
@@ -744,7 +746,7 @@ __device__ __forceinline__ void flux_net_dir(const NetW& W, float (*stash)[8][kB
         const bool fb = live && !sur_positive(lv, rv);
         const unsigned long long fm = __ballot(fb);
         list_append(fm, lane, t, fix, nfix);
-        if (stats && fm && lane == 0) atomicAdd(&stats[1], (unsigned long long)__popcll(fm));
+        if (nfb && fm && lane == 0) atomicAdd(nfb, (unsigned long long)__popcll(fm));
         if (live && !fb) sur_face_flux<D, MET>(d, c, nm, gm, i, j, k, lv, rv, F, t, nface);
     }
 }
@@ -774,15 +776,16 @@ __global__ void __launch_bounds__(kBlock) k_flux_mlp(Dims d, Coef<float> c, Prim
     if (n == 0) return;  // block-uniform: an empty segment
     load_net(W, wbf, bias);
     const size_t o = (size_t)sg * kNetSegCap;
+    unsigned long long* const nfb = stats ? sc + kCtrFb : nullptr;  // counting fallbacks
     if (dir == 0)
         flux_net_dir<1, MET>(W, stash, d, c, q, nm, gm, sl.net[0] + o, n, sl.fix[0] + o, sc + kCtrFix, sl.F[0],
-                             sl.nface[0], stats);
+                             sl.nface[0], nfb);
     else if (dir == 1)
         flux_net_dir<2, MET>(W, stash, d, c, q, nm, gm, sl.net[1] + o, n, sl.fix[1] + o, sc + kCtrFix, sl.F[1],
-                             sl.nface[1], stats);
+                             sl.nface[1], nfb);
     else
         flux_net_dir<3, MET>(W, stash, d, c, q, nm, gm, sl.net[2] + o, n, sl.fix[2] + o, sc + kCtrFix, sl.F[2],
-                             sl.nface[2], stats);
+                             sl.nface[2], nfb);
 }
 
 // PDM flux of the listed fallback faces (fp32, as the plain mixed path)
@@ -839,6 +842,37 @@ __global__ void k_zero_ctr(unsigned long long* __restrict__ p, long nseg) {
     if (t < nseg) {
         p[t * kCtrStride + kCtrNet] = 0ull;
         p[t * kCtrStride + kCtrFix] = 0ull;
+        p[t * kCtrStride + kCtrFaces] = 0ull;
+        p[t * kCtrStride + kCtrFb] = 0ull;
+    }
+}
+
+// the segments' tallies of one flux evaluation added to the context's totals
+// (faces evaluated, fallbacks); one workgroup of 1024
+__global__ void __launch_bounds__(1024) k_stats_sum(const unsigned long long* __restrict__ p, long nseg,
+                                                   unsigned long long* __restrict__ stats) {
+    __shared__ unsigned long long red[2][1024 / 64];
+    unsigned long long a = 0, b = 0;
+    for (long t = threadIdx.x; t < nseg; t += blockDim.x) {
+        a += p[t * kCtrStride + kCtrFaces];
+        b += p[t * kCtrStride + kCtrFb];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a;
+        red[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+            a += red[0][w];
+            b += red[1][w];
+        }
+        stats[0] += a;
+        stats[1] += b;
     }
 }
 
@@ -938,6 +972,7 @@ static void flux_sur_launch(mhd_ctx* c, SurrogateState* S, Src<float> s) {
     launch(c, MHD_K_SURR, dim3(mseg, 3), dim3(kBlock), sur::k_flux_mlp<MET>, c->d, co, q, nm, c->gm,
            (const unsigned short*)S->wbf, (const float*)S->bias, sl, st);
     launch(c, MHD_K_SURR, dim3(mseg, 3), dim3(kBlock), sur::k_flux_fix<MET>, c->d, co, q, nm, c->gm, sl);
+    if (st) launch(c, MHD_K_SURR, dim3(1), dim3(1024), sur::k_stats_sum, (const unsigned long long*)S->ctr, sg.segs, st);
 }
 
 void launch_flux_surrogate(mhd_ctx* c, Src<float> s) {
