@@ -182,9 +182,10 @@ void *mhd_device_pointer(mhd_ctx *ctx, int which);
  * kernel classes: see MHD_K_*; *ms = summed duration, *launches = count */
 enum { MHD_K_FLUX = 0, MHD_K_EMF = 1, MHD_K_UPDATE = 2, MHD_K_CT = 3, MHD_K_FILL = 4, MHD_K_BXYZ = 5,
        MHD_K_DT = 6, MHD_K_PREV = 7, MHD_K_SURR = 8, MHD_K_NCLASS = 9 };
-/* (the learned-reconstruction flux kernels count as MHD_K_FLUX; MHD_K_SURR
- * covers their PDM fallback fix-up, sample collection, training and
- * standalone inference) */
+/* (the learned-reconstruction flux's first pass counts as MHD_K_FLUX;
+ * MHD_K_SURR covers its MLP pass over the listed faces, the PDM fallback
+ * fix-up, the list reset, sample collection, training and standalone
+ * inference) */
 int mhd_profile(mhd_ctx *ctx, int enable);
 int mhd_profile_read(mhd_ctx *ctx, int kclass, int *launches, double *ms);
 /* time only the classes whose bit (1 << MHD_K_*) is set in mask (0: off);
