@@ -1,8 +1,8 @@
 // Learned face reconstruction (no counterpart in the reference): a small MLP
 // maps the normalised 5-point stencils of the 8 primitives at a face to the
 // face's left/right states; plain PDM (mhd3d3.f90:862-924, 1069-1172) is the
-// fallback wherever a smoothness or positivity check fails, and it is the
-// judge the net is trained against.
+// fallback wherever a smoothness or positivity check fails, and its face
+// states are the targets the net is trained on.
 //
 //   input   x[5q+k] = (f_q(cell j-2+k) - c_q)/s_q, k=0..4, face between j-1, j
 //           c_q = (f_q(j-1)+f_q(j))/2, s_q = max(max_k |f_q(j-2+k) - c_q|,
