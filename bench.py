@@ -175,6 +175,61 @@ def max_divb(st, g):
     return float(np.abs(div).max())
 
 
+FIELDS = ("rho", "vx", "vy", "vz", "p", "bi", "bj", "bk", "bx", "by", "bz")
+FACE = {"bi": 1, "bj": 2, "bk": 3}
+
+
+def field_errors(a, b, g):
+    """every output field of state a against state b over the owned cells
+    and faces: L1 = sum |a - b| / sum |b| and the largest difference over the
+    largest value of the field's group (velocity, magnetic field, or itself)"""
+    own = {k: (g.interior(a[k], FACE.get(k, 0)), g.interior(b[k], FACE.get(k, 0))) for k in FIELDS}
+    mx = {k: float(np.abs(v[1]).max()) for k, v in own.items()}
+    grp = {"v": max(mx[k] for k in ("vx", "vy", "vz")), "b": max(mx[k] for k in ("bi", "bj", "bk", "bx", "by", "bz"))}
+    out = {}
+    for k, (x, y) in own.items():
+        d = np.abs(x - y)
+        scale = grp["v"] if k[0] == "v" else (grp["b"] if k[0] == "b" else mx[k])
+        out[k] = {"L1": float(d.sum() / max(np.abs(y).sum(), 1e-300)), "max_rel": float(d.max() / max(scale, 1e-300))}
+    return out
+
+
+def accuracy(g, st, n, weights, steps):
+    """the headline's accuracy half over the configuration's full step count:
+    binary64, fp32 + learned reconstruction and fp32 + PDM from the same
+    initial state, each `steps` Adams-Bashforth steps (CFL dt), then every
+    output field of the two fp32 runs against binary64, max |div B| and the
+    state flags"""
+    runs = [("fp64", "fp64", None), ("fp32_pdm", "mixed", None)]
+    if weights is not None:
+        runs.insert(1, ("learned", "mixed", weights))
+    states, rec = {}, {"steps": steps, "state_flags": {}, "max_divB": {}}
+    for name, prec, w in runs:
+        s = gpu.Solver(gpu.make_params(n, n, n, ng=4, reconop=1, stepop=1, bc="periodic", precision=prec,
+                                       surrogate=0 if w is None else 1), g)
+        try:
+            if w is not None:
+                sg.set_weights(s, w)
+                sg.configure(s, sg.THETA, count=True)
+            s.upload(st)
+            tau = s.initialize(1e-4)
+            t0 = time.perf_counter()
+            s.step(steps, tau)
+            s.synchronize()
+            log("accuracy run %s: %d steps in %.2f s" % (name, steps, time.perf_counter() - t0))
+            rec["state_flags"][name] = s.check_state()
+            if w is not None:
+                faces, fb = sg.stats(s)
+                rec["learned_fallback_fraction"] = fb / max(faces, 1)
+            states[name] = s.download()
+        finally:
+            s.close()
+        rec["max_divB"][name] = max_divb(states[name], g)
+    rec["fields_vs_fp64"] = {name: field_errors(states[name], states["fp64"], g) for name in states if name != "fp64"}
+    rec["density_L1_vs_fp64"] = {name: e["rho"]["L1"] for name, e in rec["fields_vs_fp64"].items()}
+    return rec
+
+
 SHELL_SHAPE = (96, 64, 128)
 
 
@@ -292,21 +347,61 @@ def ensemble_config(steps, nbox=16, warmup=2):
     return rec
 
 
-def host_cores():
+def host_cpus():
+    """the CPUs this process may run on, their physical cores (hyperthread
+    siblings counted once) and the CPU model"""
     try:
-        n = len(os.sched_getaffinity(0))
+        cpus = sorted(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        cpus = list(range(os.cpu_count() or 1))
+    cores = set()
+    for c in cpus:
+        try:
+            base = "/sys/devices/system/cpu/cpu%d/topology/" % c
+            with open(base + "physical_package_id") as fh:
+                pkg = fh.read().strip()
+            with open(base + "core_id") as fh:
+                core = fh.read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add(("?", str(c)))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return len(cpus), len(cores), model
 
 
-def cpu_reference(g, st, n, nsteps, ranks):
-    """time the reference Fortran (oracle/_ref) with MPI; None if unavailable."""
+def rank_grid(p, n):
+    """xprocs, yprocs, zprocs with every factor >= 2 (the reference leaves a
+    direction's right-side ghosts unfilled with one rank, mhd3d3.f90:375-388)
+    dividing n, for the largest usable rank count <= p: powers of two, split
+    as evenly as possible"""
+    best = None
+    for a in range(1, 12):
+        for b in range(1, 12):
+            for c in range(1, 12):
+                q = (2 ** a) * (2 ** b) * (2 ** c)
+                if q <= p and all(n % (2 ** e) == 0 for e in (a, b, c)):
+                    key = (q, -max(a, b, c))
+                    if best is None or key > best[0]:
+                        best = (key, (2 ** a, 2 ** b, 2 ** c))
+    return best[1] if best else None
+
+
+def cpu_reference(g, st, n, nsteps, px, model):
+    """time the reference Fortran (oracle/_ref) with MPI on px[0]*px[1]*px[2]
+    ranks; None if unavailable."""
     exe = os.path.join(ROOT, "oracle", "_ref", "mhd3d3_ref_periodic.x")
     mpiexec = "/opt/conda/bin/mpiexec"
     if not (os.path.exists(exe) and os.path.exists(mpiexec)):
         return None
-    px = {16: (4, 2, 2), 8: (2, 2, 2)}[ranks]
+    ranks = px[0] * px[1] * px[2]
     run = tempfile.mkdtemp(prefix="mhdref_bench_")
     try:
         with open(os.path.join(run, "ic.bin"), "wb") as fh:
@@ -331,8 +426,10 @@ def cpu_reference(g, st, n, nsteps, ranks):
             if "STEPLOOP_SECONDS" in line:
                 t = float(line.split()[-1])
                 return dict(value=n ** 3 * nsteps / t, unit="zone-updates/s", cores=ranks, kind="reference",
-                            sample="%d^3 blast wave on the reference's Cartesian grid, PDM AB2, %d steps, mpiexec -n %d (%dx%dx%d), step loop "
-                                   "timed by MPI_Wtime" % (n, nsteps, ranks, px[0], px[1], px[2]))
+                            cpu_model=model, step_loop_s=t,
+                            sample="%d^3 blast wave on the reference's Cartesian grid, PDM AB2, %d steps, mpiexec -n %d "
+                                   "(%dx%dx%d, one rank per physical core), step loop timed by MPI_Wtime; flang -O3 "
+                                   "-fdefault-real-8, no fast-math" % (n, nsteps, ranks, px[0], px[1], px[2]))
         return None
     except (subprocess.TimeoutExpired, OSError) as e:
         log("reference run failed:", e)
@@ -341,7 +438,7 @@ def cpu_reference(g, st, n, nsteps, ranks):
         shutil.rmtree(run, ignore_errors=True)
 
 
-def cpu_port(g, st, n, nsteps, threads):
+def cpu_port(g, st, n, nsteps, threads, model):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     po.lib().orc_set_threads(threads)
@@ -351,7 +448,7 @@ def cpu_port(g, st, n, nsteps, threads):
     t0 = time.perf_counter()
     o.step(nsteps, tau)
     t = time.perf_counter() - t0
-    return dict(value=n ** 3 * nsteps / t, unit="zone-updates/s", cores=threads, kind="port",
+    return dict(value=n ** 3 * nsteps / t, unit="zone-updates/s", cores=threads, kind="port", cpu_model=model,
                 sample="%d^3 blast wave on the Cartesian grid, PDM AB2, %d steps, C oracle with %d OpenMP threads" % (n, nsteps, threads))
 
 
@@ -362,7 +459,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--accuracy-steps", type=int, default=500,
+                    help="steps of the accuracy comparison (binary64 vs learned vs fp32 + PDM), apart from the "
+                         "timed steps; 0: skip it")
     ap.add_argument("--only", choices=["plain", "fast", "both"], default="both")
     ap.add_argument("--no-surrogate", action="store_true", help="fast run without the learned reconstruction")
     ap.add_argument("--weights", help="float32 file of trained weights (skips training)")
@@ -462,13 +562,21 @@ def main():
         out["plain_fp64"] = {"value": pv, "ms_per_step": res["plain"]["wall"] / K * 1e3,
                              "max_divB": max_divb(res["plain"]["state"], g)}
         out["speedup_vs_plain_fp64"] = value / pv
-        out["density_L1_vs_plain_fp64"] = l1_rel(res["fast"]["state"], res["plain"]["state"], g)
+        out["density_L1_after_timed_steps"] = l1_rel(res["fast"]["state"], res["plain"]["state"], g)
         out["max_divB"] = max_divb(res["fast"]["state"], g)
+        if args.accuracy_steps > 0:
+            log("accuracy comparison over %d steps ..." % args.accuracy_steps)
+            acc = accuracy(g, st, n, weights, args.accuracy_steps)
+            out["accuracy"] = acc
+            key = "learned" if sur else "fp32_pdm"
+            out["density_L1_vs_plain_fp64"] = acc["density_L1_vs_fp64"][key]
+            out["density_L1_steps"] = args.accuracy_steps
+            out["max_divB"] = acc["max_divB"][key]
     if "mixed_pdm" in res and "plain" in res:
         r = res["mixed_pdm"]
         out["mixed_fp32_pdm"] = {"value": n ** 3 * K / r["wall"], "ms_per_step": r["wall"] / K * 1e3,
                                  "speedup_vs_plain_fp64": res["plain"]["wall"] / r["wall"],
-                                 "density_L1_vs_plain_fp64": l1_rel(r["state"], res["plain"]["state"], g),
+                                 "density_L1_after_timed_steps": l1_rel(r["state"], res["plain"]["state"], g),
                                  "max_divB": max_divb(r["state"], g),
                                  "kernel_ms_per_step": {k: v[1] / K for k, v in r["prof"].items() if v[0]}}
     out["kernel_ms_per_step"] = {k: v[1] / K for k, v in head["prof"].items() if v[0]}
@@ -489,15 +597,17 @@ def main():
         log("batched-ensemble record (%d steps) ..." % args.ensemble_steps)
         out["ensemble_config"] = ensemble_config(args.ensemble_steps)
     if not args.no_cpu_baseline:
-        cores = host_cores()
-        ranks = 16 if cores >= 16 else 8
-        log("cpu baseline: reference with %d MPI ranks ..." % ranks)
+        ncpu, ncore, model = host_cpus()
+        px = rank_grid(ncore, n)
+        log("cpu baseline: reference on %s MPI ranks (%d CPUs, %d physical cores: %s) ..." % (px, ncpu, ncore, model))
         gc, stc = workload(n, sheared=False)   # the reference's own (Cartesian) geometry
-        cb = cpu_reference(gc, stc, n, args.cpu_steps, ranks)
+        cb = cpu_reference(gc, stc, n, args.cpu_steps, px, model) if px else None
         if cb is None:
             m = 64
             g2, st2 = workload(m, sheared=False)
-            cb = cpu_port(g2, st2, m, 2, cores)
+            cb = cpu_port(g2, st2, m, 10, ncore, model)
+        cb["host_cpus"] = ncpu
+        cb["host_physical_cores"] = ncore
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 

@@ -141,12 +141,58 @@ int mhd_ensemble_create(mhd_ens **ens, mhd_ctx *const *members, int n, const dou
 int mhd_ensemble_step(mhd_ens *ens, int nsteps);
 void mhd_ensemble_destroy(mhd_ens *ens);
 
-/* per-routine drop-ins, each acting on the context's current state */
-int mhd_get_dt(mhd_ctx *ctx, double *tau_cfl);   /* get_dt, mhd3d3.f90:1608 */
+/* per-routine drop-ins.  With them a caller runs the reference's step
+ * routine by routine, e.g. step_rk2 (mhd3d3.f90:3648-3711):
+ *     mhd_get_dt; mhd_set_tau(tau0, tau0);
+ *     for stage 1, 2: mhd_eijk_lfm(CURRENT); mhd_flux_stress(CURRENT);
+ *                     mhd_advance(stage); mhd_fill_ghosts; mhd_get_bxyz
+ * and step_ab (3602-3644):
+ *     mhd_get_dt; mhd_predict_ab; mhd_eijk_lfm(PREDICTED);
+ *     mhd_flux_stress(PREDICTED); mhd_advance(0); mhd_fill_ghosts; mhd_get_bxyz
+ * which reproduces mhd_step bit for bit. */
+
+/* get_dt, mhd3d3.f90:1608-1639: *tau_cfl = the CFL step, min over the
+ * ghost-inclusive block; like the reference it also sets the context's tau */
+int mhd_get_dt(mhd_ctx *ctx, double *tau_cfl);
 int mhd_fill_ghosts(mhd_ctx *ctx);                /* fill_hydro_mpi + fill_mag_mpi, 781-819 */
 int mhd_get_bxyz(mhd_ctx *ctx);                   /* get_bxyz, 2118 */
-int mhd_flux_stress(mhd_ctx *ctx);                /* get_flux_stress, 2385 */
-int mhd_eijk_lfm(mhd_ctx *ctx);                   /* get_eijk_LFM, 1754 */
+
+/* the state a routine reads: the reference passes it as the array
+ * arguments of get_flux_stress / get_eijk_LFM -- the current state in
+ * step_rk2/rk3 (mhd3d3.f90:3665-3667), the Adams-Bashforth extrapolation
+ * rhoh, vxh, ... in step_ab (3618-3620).  PREDICTED needs an
+ * Adams-Bashforth context whose mhd_predict_ab is the last routine that
+ * changed the state (MHD_ERR_PARAM otherwise). */
+enum { MHD_STATE_CURRENT = 0, MHD_STATE_PREDICTED = 1 };
+int mhd_flux_stress(mhd_ctx *ctx, int which_state);   /* get_flux_stress, 2385-2560 */
+int mhd_eijk_lfm(mhd_ctx *ctx, int which_state);      /* get_eijk_LFM, 1754-1856 */
+
+/* the module variables tau and tau0 (module pars, mhd3d3.f90:33-248): the
+ * step the update routines use and the previous one (both > 0); e.g. the
+ * reference's `tau=tau0` of step_rk2 (3662) */
+int mhd_set_tau(mhd_ctx *ctx, double tau, double tau0);
+int mhd_get_tau(mhd_ctx *ctx, double *tau, double *tau0);
+
+/* advance_predictor_ab (mhd3d3.f90:3821-3840) with its advance_previous and
+ * the tau0=tau that follows it in step_ab (3616): the extrapolated state
+ * x + tau/tau0/2 (x - x_p) of the 11 state arrays, then x_p = x.
+ * Adams-Bashforth contexts only (MHD_ERR_UNSUPPORTED otherwise). */
+int mhd_predict_ab(mhd_ctx *ctx);
+
+/* the update of one stage from the face fluxes and edge EMFs the context
+ * holds (mhd_flux_stress / mhd_eijk_lfm, or mhd_store): get_conserve
+ * (1641-1650), advance_corrector1 (3842-3878), the stage combine and
+ * pressure, advance_corrector2 (3880-3907), the combine and velocity,
+ * advance_bijk (3909-3933) and the face combine -- the body of
+ * step_rk2/step_rk3 (3665-3707, 3733-3785) or step_ab (3622-3627) between
+ * the flux and the ghost fill, in place on the current state.
+ *   stage 0: Adams-Bashforth corrector (stepop 1);
+ *   stage 1..stepop: Runge-Kutta stage; stage 1 first takes the
+ *            advance_previous copies (3794-3819) of the state it starts
+ *            from (step_rk2 takes them just before stage 1, 3664).
+ * Cell B and the CFL partials follow at mhd_get_bxyz, as in the
+ * reference.  MHD_ERR_PARAM for a stage the context's stepper lacks. */
+int mhd_advance(mhd_ctx *ctx, int stage);
 
 /* copy a device product to the host:
  *   which = MHD_FETCH_FLUX + q (q = 8*d + f, d = face direction x/y/z,
@@ -154,11 +200,24 @@ int mhd_eijk_lfm(mhd_ctx *ctx);                   /* get_eijk_LFM, 1754 */
  *           MHD_FETCH_EI/EJ/EK, MHD_FETCH_VF (cell array |v|+|B|/sqrt(rho)) */
 enum { MHD_FETCH_FLUX = 0, MHD_FETCH_EI = 24, MHD_FETCH_EJ = 25, MHD_FETCH_EK = 26, MHD_FETCH_VF = 27 };
 int mhd_fetch(mhd_ctx *ctx, int which, double *host_out);
+/* the inverse of mhd_fetch for the face fluxes and edge EMFs (no
+ * counterpart in the reference, whose flux arrays are module variables any
+ * routine may write, mhd3d3.f90:273-280): the next mhd_advance uses them.
+ * Mixed precision stores them as fp32.  MHD_ERR_PARAM for other products. */
+int mhd_store(mhd_ctx *ctx, int which, const double *host_in);
 
 /* reconstruct_3d (mhd3d3.f90:833-860) of a host array f(fn1,fn2,fn3) into
  * fl, fr (lnx,lny,lnz); dir 1-3 truncated, 4-6 untruncated transverse */
 int mhd_reconstruct(mhd_ctx *ctx, int dir, const double *f, int fn1, int fn2, int fn3, int lnx, int lny,
                     int lnz, double *fl, double *fr);
+
+/* get_v_face (mhd3d3.f90:2594-2615) elementwise on n host values, in the
+ * arithmetic of the context's flux: binary64 (ROCm erfc/exp) or, in mixed
+ * precision, fp32 (erfc by Abramowitz & Stegun 7.1.26 sharing the thermal
+ * exponential).  v0p = erfc(-a v)/2, v0n = erfc(a v)/2, v1p = v v0p + b,
+ * v1n = v v0n - b, a = sqrt(lam), b = exp(-lam v^2)/(2 a sqrt(pi)). */
+int mhd_v_face(mhd_ctx *ctx, long n, const double *lam, const double *v, double *v0p, double *v0n, double *v1p,
+               double *v1n);
 
 /* divergence monitor (no counterpart in the reference, which has no div B
  * diagnostic): *divb = max over the owned cells of |div B| of the current

@@ -124,6 +124,7 @@ int mhd_create(mhd_ctx** out, const mhd_params* p) {
         plan.push_back({&c->emf[a], align_up(c->ne[a] * tf)});
     }
     plan.push_back({(void**)&c->dtpart, align_up(c->ndtpart * sizeof(double))});
+    plan.push_back({(void**)&c->divpart, align_up(blocks(c->ninterior).x * sizeof(double))});
     plan.push_back({(void**)&c->scal, align_up(8 * sizeof(double))});
     plan.push_back({(void**)&c->flags, align_up(64)});
     const bool mixed = p->precision == MHD_MIXED;
@@ -556,7 +557,10 @@ int mhd_upload_state(mhd_ctx* c, const double* rho, const double* vx, const doub
                           c->nbk * sizeof(double)};
     for (int a = 0; a < 8; ++a)
         if (hipMemcpyAsync(dst[a], src[a], nb[a], hipMemcpyHostToDevice, c->s) != hipSuccess) return MHD_ERR_HIP;
+    // cell B of the new faces is formed where it is read (get_dt, downloads)
     c->dt_fresh = false;
+    c->cellb_stale = true;
+    c->pred_valid = false;
     return hipStreamSynchronize(c->s) == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
@@ -685,11 +689,16 @@ static void do_bxyz(mhd_ctx* c, bool want_vf, bool want_shadow, bool want_dt, bo
     c->dt_fresh = want_dt;
     c->ndt_valid = blocks(c->nmax).x;
     c->cellb_stale = !store_cellb;
+    // mixed precision keeps the extrapolated state in the fp32 copies
+    if (want_shadow && mixed(c)) c->pred_valid = false;
 }
 
 // get_dt on the current state; the minimum lands in scal[SC_DTCFL] and, when
 // set_tau, also in scal[SC_TAU] (Adams-Bashforth takes tau from the CFL)
 static void do_dt(mhd_ctx* c, bool set_tau) {
+    // where cell B of the current state is not stored (mixed Adams-Bashforth
+    // steps, uploads), get_bxyz forms it and the CFL partials in one pass
+    if (!c->dt_fresh && c->cellb_stale) do_bxyz(c, false, false, true);
     if (!c->dt_fresh) {
         launch(c, MHD_K_DT, blocks(c->ncell), dim3(kBlock), k_dt_partial, c->d, c->p.gam, c->p.cfl, cur_state(c),
                sdx(c), sdy(c), sdz(c), gmet(c), c->dtpart, c->ncell);
@@ -710,14 +719,14 @@ static void do_prev_rk(mhd_ctx* c) {
 // conserved update from the current state into `out` (the current state
 // itself for RK, the next-state buffers for AB)
 template <typename TF, int NST, int ST>
-static void do_cells_t(mhd_ctx* c, double* const* out) {
+static void do_cells_t(mhd_ctx* c, double* const* out, bool fused_ok = true) {
     CCells s{c->rho, c->vx, c->vy, c->vz, c->pr};
     Cells so{out[0], out[1], out[2], out[3], out[4]};
     PrevRK pv{c->rhop, c->mxp, c->myp, c->mzp, c->engp};
     FluxSet<TF> F{(const TF*)c->flux[0], (const TF*)c->flux[1], (const TF*)c->flux[2], c->nf[0], c->nf[1], c->nf[2]};
     const bool gen = c->met.mode == 2;
     if constexpr (NST == 0 && std::is_same_v<TF, float>) {
-        if (fuse_dt(c)) {  // the CFL partials of the new state too (CT has run)
+        if (fused_ok && fuse_dt(c)) {  // the CFL partials of the new state too (CT has run)
             UpdDt u{};
             u.bi = out[8];
             u.bj = out[9];
@@ -839,8 +848,7 @@ int mhd_get_bxyz(mhd_ctx* c) {
 int mhd_get_dt(mhd_ctx* c, double* tau_cfl) {
     if (!c || !c->have_geometry) return MHD_ERR_PARAM;
     hipSetDevice(c->p.device);
-    if (!c->dt_fresh) ensure_cellb(c);
-    do_dt(c, false);
+    do_dt(c, true);   // get_dt sets the module's tau (mhd3d3.f90:1629)
     double h = 0.;
     if (hipMemcpyAsync(&h, c->scal + SC_DTCFL, sizeof(double), hipMemcpyDeviceToHost, c->s) != hipSuccess)
         return MHD_ERR_HIP;
@@ -849,26 +857,43 @@ int mhd_get_dt(mhd_ctx* c, double* tau_cfl) {
     return rc;
 }
 
-int mhd_flux_stress(mhd_ctx* c) {
+// which_state: the state the reference passes as the routines' array
+// arguments -- the current one (step_rk2/rk3, mhd3d3.f90:3665-3667) or the
+// Adams-Bashforth extrapolation of mhd_predict_ab (step_ab, 3618-3620)
+static int check_which(const mhd_ctx* c, int which) {
+    if (which == MHD_STATE_CURRENT) return MHD_OK;
+    if (which != MHD_STATE_PREDICTED) return MHD_ERR_PARAM;
+    return c->p.stepop == 1 && c->pred_valid ? MHD_OK : MHD_ERR_PARAM;
+}
+
+int mhd_flux_stress(mhd_ctx* c, int which_state) {
     if (!c || !c->have_geometry) return MHD_ERR_PARAM;
+    if (int rc = check_which(c, which_state)) return rc;
     hipSetDevice(c->p.device);
+    const bool pred = which_state == MHD_STATE_PREDICTED;
     if (mixed(c)) {
-        do_bxyz(c, false, true, false);
+        // the extrapolated state already sits in the fp32 copies
+        if (!pred) do_bxyz(c, false, true, false);
         if (c->p.surrogate) launch_flux_surrogate(c, sh_src(c));
         else launch_flux_fp32(c, sh_src(c));
     } else {
-        launch_flux_fp64(c, cur_src(c));
+        if (!pred) ensure_cellb(c);
+        launch_flux_fp64(c, pred ? h_src(c) : cur_src(c));
     }
     return sync_status(c);
 }
 
-int mhd_eijk_lfm(mhd_ctx* c) {
+int mhd_eijk_lfm(mhd_ctx* c, int which_state) {
     if (!c || !c->have_geometry) return MHD_ERR_PARAM;
+    if (int rc = check_which(c, which_state)) return rc;
     hipSetDevice(c->p.device);
     if (mixed(c)) {
-        do_bxyz(c, false, true, false);
+        if (which_state != MHD_STATE_PREDICTED) do_bxyz(c, false, true, false);
         launch_emf_fp32(c, sh_src(c));
+    } else if (which_state == MHD_STATE_PREDICTED) {
+        launch_emf_fp64(c, h_src(c));   // vf and face-normal velocities written by the predictor
     } else {
+        ensure_cellb(c);
         if (c->met.on) do_bxyz(c, true, false, false);   // vf and the face-normal velocities
         else launch(c, MHD_K_BXYZ, blocks(c->ncell), dim3(kBlock), k_vf, cur_state(c), c->vf, c->ncell);
         launch_emf_fp64(c, cur_src(c));
@@ -895,6 +920,7 @@ int mhd_initialize(mhd_ctx* c, double* tau) {
     hipSetDevice(c->p.device);
     const double t = tau ? *tau : 0.;
     c->tau0_host = t;
+    c->pred_valid = false;
     launch(c, MHD_K_PREV, dim3(1), dim3(64), k_set2, c->scal + SC_TAU, t, t);
     do_fill(c);
     end_of_stage(c);
@@ -935,8 +961,10 @@ static void step_rk(mhd_ctx* c) {
 // step_ab (mhd3d3.f90:3602-3644): the new state goes to the next-state
 // buffers, then (previous, current, next) rotate -- the reference's
 // advance_previous copy x_p = x without moving data
-static void step_ab(mhd_ctx* c) {
-    do_dt(c, true);
+// advance_predictor_ab (mhd3d3.f90:3821-3840) up to its advance_previous:
+// the extrapolated state into xh (binary64) or the fp32 stencil copies
+// (mixed), then tau0 = tau (3616)
+static void predict_ab(mhd_ctx* c) {
     const long nc = c->ncell, nbi = c->nbi, nbj = c->nbj, nbk = c->nbk;
     if (mixed(c)) {
         AB11<float> s;
@@ -966,6 +994,11 @@ static void step_ab(mhd_ctx* c) {
                (const double*)(c->scal + SC_TAU), c->vfh, pj, nc, nbi, nbj, nbk);
         launch(c, MHD_K_PREV, dim3(1), dim3(64), k_tau0_from_tau, c->scal + SC_TAU);
     }
+}
+
+static void step_ab(mhd_ctx* c) {
+    do_dt(c, true);
+    predict_ab(c);
     do_stage<0, 0>(c, h_src(c), c->xn);
     // rotate: previous <- current <- next <- previous
     for (int a = 0; a < 11; ++a) {
@@ -988,6 +1021,7 @@ int mhd_step(mhd_ctx* c, int nsteps, double* tau, double* tau_cfl) {
     if (!c || nsteps < 0 || !c->have_geometry) return MHD_ERR_PARAM;
     hipSetDevice(c->p.device);
     if (c->p.stepop != 1 && tau) c->tau0_host = *tau;
+    if (nsteps > 0) c->pred_valid = false;
     for (int n = 0; n < nsteps; ++n) {
         if (c->p.stepop == 1) step_ab(c);
         else if (c->p.stepop == 2) step_rk<2>(c);
@@ -1000,6 +1034,108 @@ int mhd_step(mhd_ctx* c, int nsteps, double* tau, double* tau_cfl) {
     if (tau) *tau = h[SC_TAU];
     if (tau_cfl) *tau_cfl = h[SC_DTCFL];
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// per-routine drop-ins of the time stepper (the calls step_ab / step_rk2 /
+// step_rk3 make between get_dt and the ghost fill)
+// ---------------------------------------------------------------------------
+int mhd_set_tau(mhd_ctx* c, double tau, double tau0) {
+    if (!c || !(tau > 0.) || !(tau0 > 0.) || !std::isfinite(tau) || !std::isfinite(tau0)) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    launch(c, MHD_K_PREV, dim3(1), dim3(64), k_set2, c->scal + SC_TAU, tau, tau0);
+    c->tau0_host = tau0;
+    return sync_status(c);
+}
+
+int mhd_get_tau(mhd_ctx* c, double* tau, double* tau0) {
+    if (!c) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    double h[2] = {0., 0.};
+    if (hipMemcpyAsync(h, c->scal + SC_TAU, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s) != hipSuccess)
+        return MHD_ERR_HIP;
+    const int rc = sync_status(c);
+    if (tau) *tau = h[0];
+    if (tau0) *tau0 = h[1];
+    return rc;
+}
+
+int mhd_predict_ab(mhd_ctx* c) {
+    if (!c || !c->have_geometry) return MHD_ERR_PARAM;
+    if (c->p.stepop != 1) return MHD_ERR_UNSUPPORTED;
+    hipSetDevice(c->p.device);
+    if (!mixed(c)) ensure_cellb(c);   // the binary64 predictor extrapolates cell B too
+    predict_ab(c);
+    // advance_previous (3794-3819, Adams-Bashforth flavour): x_p = x
+    for (int a = 0; a < 11; ++a) {
+        const long n = a < 8 ? c->ncell : (a == 8 ? c->nbi : (a == 9 ? c->nbj : c->nbk));
+        if (hipMemcpyAsync(c->xp[a], *state_slot(c, a), n * sizeof(double), hipMemcpyDeviceToDevice, c->s) !=
+            hipSuccess)
+            return MHD_ERR_HIP;
+    }
+    const int rc = sync_status(c);
+    c->pred_valid = rc == MHD_OK;
+    return rc;
+}
+
+// one stage's update in place, in the reference's serial order
+template <int NST, int ST>
+static void advance_serial(mhd_ctx* c) {
+    double* self[11];
+    for (int a = 0; a < 11; ++a) self[a] = *state_slot(c, a);
+    if (mixed(c)) {
+        do_cells_t<float, NST, ST>(c, self, false);
+        do_ct_t<float, NST, ST>(c, self);
+    } else {
+        do_cells_t<double, NST, ST>(c, self, false);
+        do_ct_t<double, NST, ST>(c, self);
+    }
+}
+
+int mhd_advance(mhd_ctx* c, int stage) {
+    if (!c || !c->have_geometry) return MHD_ERR_PARAM;
+    const int nst = c->p.stepop == 1 ? 0 : c->p.stepop;
+    if (nst == 0 ? stage != 0 : (stage < 1 || stage > nst)) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    if (stage == 1) do_prev_rk(c);   // advance_previous: the state stage 1 starts from
+    if (nst == 0) advance_serial<0, 0>(c);
+    else if (nst == 2 && stage == 1) advance_serial<2, 1>(c);
+    else if (nst == 2) advance_serial<2, 2>(c);
+    else if (stage == 1) advance_serial<3, 1>(c);
+    else if (stage == 2) advance_serial<3, 2>(c);
+    else advance_serial<3, 3>(c);
+    // the reference's cell B and CFL partials follow at get_bxyz
+    c->dt_fresh = false;
+    c->cellb_stale = true;
+    c->pred_valid = false;
+    return sync_status(c);
+}
+
+int mhd_store(mhd_ctx* c, int which, const double* host_in) {
+    if (!c || !host_in) return MHD_ERR_PARAM;
+    hipSetDevice(c->p.device);
+    const bool f32 = mixed(c);
+    void* dst;
+    long n;
+    if (which >= MHD_FETCH_FLUX && which < MHD_FETCH_FLUX + 24) {
+        const int dq = which / 8, fq = which % 8;
+        n = c->nf[dq];
+        dst = (char*)c->flux[dq] + (size_t)fq * n * (f32 ? 4 : 8);
+    } else if (which >= MHD_FETCH_EI && which <= MHD_FETCH_EK) {
+        n = c->ne[which - MHD_FETCH_EI];
+        dst = c->emf[which - MHD_FETCH_EI];
+    } else {
+        return MHD_ERR_PARAM;
+    }
+    if (f32) {
+        std::vector<float> tmp(host_in, host_in + n);
+        if (hipMemcpyAsync(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, c->s) != hipSuccess)
+            return MHD_ERR_HIP;
+        return sync_status(c);
+    }
+    if (hipMemcpyAsync(dst, host_in, n * sizeof(double), hipMemcpyHostToDevice, c->s) != hipSuccess)
+        return MHD_ERR_HIP;
+    return sync_status(c);
 }
 
 // ---------------------------------------------------------------------------
@@ -1175,16 +1311,41 @@ int mhd_reconstruct(mhd_ctx* c, int dir, const double* f, int fn1, int fn2, int 
     return rc;
 }
 
+int mhd_v_face(mhd_ctx* c, long n, const double* lam, const double* v, double* v0p, double* v0n, double* v1p,
+               double* v1n) {
+    if (!c || n < 0 || (n > 0 && (!lam || !v || !v0p || !v0n || !v1p || !v1n))) return MHD_ERR_PARAM;
+    if (n == 0) return MHD_OK;
+    hipSetDevice(c->p.device);
+    double* buf = nullptr;
+    const size_t b = (size_t)n * sizeof(double);
+    if (hipMalloc(&buf, 6 * b) != hipSuccess) return MHD_ERR_HIP;
+    double* d[6] = {buf, buf + n, buf + 2 * n, buf + 3 * n, buf + 4 * n, buf + 5 * n};
+    int rc = MHD_OK;
+    if (hipMemcpyAsync(d[0], lam, b, hipMemcpyHostToDevice, c->s) != hipSuccess ||
+        hipMemcpyAsync(d[1], v, b, hipMemcpyHostToDevice, c->s) != hipSuccess)
+        rc = MHD_ERR_HIP;
+    if (rc == MHD_OK) {
+        if (mixed(c)) launch_vface_fp32(c, n, d[0], d[1], d[2], d[3], d[4], d[5]);
+        else launch_vface_fp64(c, n, d[0], d[1], d[2], d[3], d[4], d[5]);
+        double* out[4] = {v0p, v0n, v1p, v1n};
+        for (int q = 0; q < 4 && rc == MHD_OK; ++q)
+            if (hipMemcpyAsync(out[q], d[2 + q], b, hipMemcpyDeviceToHost, c->s) != hipSuccess) rc = MHD_ERR_HIP;
+        const int r2 = sync_status(c);
+        if (rc == MHD_OK) rc = r2;
+    }
+    hipFree(buf);
+    return rc;
+}
+
 int mhd_divb_max(mhd_ctx* c, double* divb) {
     if (!c || !divb || !c->have_geometry) return MHD_ERR_PARAM;
     hipSetDevice(c->p.device);
     const dim3 g = blocks(c->ninterior);
     auto k = c->met.mode == 2 ? k_divb_partial<true> : k_divb_partial<false>;
-    // the CFL partials' buffer serves as scratch: the next get_dt recomputes them
+    // partials of its own: the CFL partials a step may have left stay valid
     launch(c, MHD_K_DT, g, dim3(kBlock), k, c->d, CFaces{c->bi, c->bj, c->bk}, sdx(c), sdy(c), sdz(c), gmet(c),
-           c->dtpart, c->ninterior);
-    c->dt_fresh = false;
-    launch(c, MHD_K_DT, dim3(1), dim3(1024), k_dt_final, (const double*)c->dtpart, (long)g.x, c->scal + SC_DIVB,
+           c->divpart, c->ninterior);
+    launch(c, MHD_K_DT, dim3(1), dim3(1024), k_dt_final, (const double*)c->divpart, (long)g.x, c->scal + SC_DIVB,
            (double*)nullptr);
     double h = 0.;
     if (hipMemcpyAsync(&h, c->scal + SC_DIVB, sizeof(double), hipMemcpyDeviceToHost, c->s) != hipSuccess)

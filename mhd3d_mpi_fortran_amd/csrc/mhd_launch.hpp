@@ -84,11 +84,13 @@ struct mhd_ctx {
     bool dt_fresh = false;   // dtpart holds the CFL partials of the current state
     long ndt_valid = 0;      // ... that many of them
     bool ghost_twins = false;  // every ghost cell repeats an owned cell (periodic axes, equal ghost spacings)
-    bool cellb_stale = false;  // bx,by,bz not stored for the current state (mixed AB steps)
+    bool cellb_stale = false;  // bx,by,bz not stored for the current state (mixed AB steps, uploads, advances)
+    bool pred_valid = false;   // the extrapolated state of mhd_predict_ab is held (xh / fp32 copies)
     // per-stage products (binary64, or fp32 in mixed precision)
     void* flux[3] = {nullptr, nullptr, nullptr};
     void* emf[3] = {nullptr, nullptr, nullptr};
     double* dtpart = nullptr;
+    double* divpart = nullptr;   // per-workgroup partials of the div B monitor (apart from the CFL ones)
     double* scal = nullptr;
     int* flags = nullptr;
     double tau0_host = 0.;
@@ -184,4 +186,8 @@ void launch_recon(mhd_ctx* c, int dir, const double* f, int fn1, int fn2, int ln
 void launch_flux_fp32(mhd_ctx* c, mhd::Src<float> s);
 void launch_emf_fp32(mhd_ctx* c, mhd::Src<float> s);
 void launch_flux_surrogate(mhd_ctx* c, mhd::Src<float> s);
+void launch_vface_fp64(mhd_ctx* c, long n, const double* lam, const double* v, double* v0p, double* v0n, double* v1p,
+                       double* v1n);
+void launch_vface_fp32(mhd_ctx* c, long n, const double* lam, const double* v, double* v0p, double* v0n, double* v1p,
+                       double* v1n);
 void surrogate_free(mhd_ctx* c);

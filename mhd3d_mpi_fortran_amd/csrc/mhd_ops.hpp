@@ -96,4 +96,28 @@ void emf_launch(mhd_ctx* c, Src<TI> s) {
     }
 }
 
+// get_v_face (mhd3d3.f90:2594-2615) elementwise in the arithmetic type R of
+// the context's flux (binary64 inputs and outputs)
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_vface(long n, const double* __restrict__ lam,
+                                                 const double* __restrict__ v, double* __restrict__ v0p,
+                                                 double* __restrict__ v0n, double* __restrict__ v1p,
+                                                 double* __restrict__ v1n, R rspi) {
+    const long t = gtid();
+    if (t >= n) return;
+    R a0, a1, b0, b1;
+    vface<R>(R(lam[t]), R(v[t]), +1, rspi, a0, a1);
+    vface<R>(R(lam[t]), R(v[t]), -1, rspi, b0, b1);
+    v0p[t] = double(a0);
+    v1p[t] = double(a1);
+    v0n[t] = double(b0);
+    v1n[t] = double(b1);
+}
+
+template <typename R>
+void vface_launch(mhd_ctx* c, long n, const double* lam, const double* v, double* v0p, double* v0n, double* v1p,
+                  double* v1n) {
+    launch(c, MHD_K_FLUX, blocks(n), dim3(kBlock), k_vface<R>, n, lam, v, v0p, v0n, v1p, v1n, make_coef<R>(c).rspi);
+}
+
 }  // namespace mhd

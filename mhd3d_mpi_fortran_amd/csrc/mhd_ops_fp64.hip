@@ -32,3 +32,8 @@ void launch_recon(mhd_ctx* c, int dir, const double* f, int fn1, int fn2, int ln
         break;
     }
 }
+
+void launch_vface_fp64(mhd_ctx* c, long n, const double* lam, const double* v, double* v0p, double* v0n, double* v1p,
+                     double* v1n) {
+    vface_launch<double>(c, n, lam, v, v0p, v0n, v1p, v1n);
+}

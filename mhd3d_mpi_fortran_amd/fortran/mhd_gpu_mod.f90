@@ -16,6 +16,8 @@ module mhd_gpu
                                 MHD_ERR_STATE = 3, MHD_ERR_UNSUPPORTED = 4
    integer(c_int), parameter :: MHD_BC_EVEN = 0, MHD_BC_PERIODIC = 2
    integer(c_int), parameter :: MHD_FP64 = 0, MHD_MIXED = 1
+   ! the state get_flux_stress / get_eijk_LFM read (their array arguments)
+   integer(c_int), parameter :: MHD_STATE_CURRENT = 0, MHD_STATE_PREDICTED = 1
 
    type, bind(C) :: mhd_params
       integer(c_int) :: nx0, ny0, nz0, ng
@@ -86,13 +88,43 @@ module mhd_gpu
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
       end function
-      integer(c_int) function mhd_flux_stress(ctx) bind(C, name='mhd_flux_stress')
+      integer(c_int) function mhd_flux_stress(ctx, which_state) bind(C, name='mhd_flux_stress')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: which_state
+      end function
+      integer(c_int) function mhd_eijk_lfm(ctx, which_state) bind(C, name='mhd_eijk_lfm')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: which_state
+      end function
+      ! the module variables tau, tau0
+      integer(c_int) function mhd_set_tau(ctx, tau, tau0) bind(C, name='mhd_set_tau')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), value :: tau, tau0
+      end function
+      integer(c_int) function mhd_get_tau(ctx, tau, tau0) bind(C, name='mhd_get_tau')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: tau, tau0
+      end function
+      ! advance_predictor_ab (+ advance_previous, tau0=tau)
+      integer(c_int) function mhd_predict_ab(ctx) bind(C, name='mhd_predict_ab')
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
       end function
-      integer(c_int) function mhd_eijk_lfm(ctx) bind(C, name='mhd_eijk_lfm')
+      ! advance_corrector1/2 + stage combine + advance_bijk (stage 0: AB)
+      integer(c_int) function mhd_advance(ctx, stage) bind(C, name='mhd_advance')
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
+         integer(c_int), value :: stage
+      end function
+      integer(c_int) function mhd_store(ctx, which, a) bind(C, name='mhd_store')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: which
+         real(c_double), intent(in) :: a(*)
       end function
       integer(c_int) function mhd_fetch(ctx, which, out) bind(C, name='mhd_fetch')
          import :: c_int, c_ptr, c_double

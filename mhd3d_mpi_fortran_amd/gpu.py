@@ -21,6 +21,7 @@ BC = {"even": 0, "periodic": 2}
 PRECISION = {"fp64": 0, "mixed": 1}
 KCLASS = {"flux": 0, "emf": 1, "update": 2, "ct": 3, "fill": 4, "bxyz": 5, "dt": 6, "prev": 7, "surrogate": 8}
 FETCH_EI, FETCH_EJ, FETCH_EK, FETCH_VF = 24, 25, 26, 27
+STATE_CURRENT, STATE_PREDICTED = 0, 1
 
 
 class MhdError(RuntimeError):
@@ -57,9 +58,15 @@ SIGNATURES = {
     "mhd_get_dt": (C.c_int, [C.c_void_p, _D]),
     "mhd_fill_ghosts": (C.c_int, [C.c_void_p]),
     "mhd_get_bxyz": (C.c_int, [C.c_void_p]),
-    "mhd_flux_stress": (C.c_int, [C.c_void_p]),
-    "mhd_eijk_lfm": (C.c_int, [C.c_void_p]),
+    "mhd_flux_stress": (C.c_int, [C.c_void_p, C.c_int]),
+    "mhd_eijk_lfm": (C.c_int, [C.c_void_p, C.c_int]),
+    "mhd_set_tau": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "mhd_get_tau": (C.c_int, [C.c_void_p, _D, _D]),
+    "mhd_predict_ab": (C.c_int, [C.c_void_p]),
+    "mhd_advance": (C.c_int, [C.c_void_p, C.c_int]),
     "mhd_fetch": (C.c_int, [C.c_void_p, C.c_int, _D]),
+    "mhd_store": (C.c_int, [C.c_void_p, C.c_int, _D]),
+    "mhd_v_face": (C.c_int, [C.c_void_p, C.c_long, _D, _D, _D, _D, _D, _D]),
     "mhd_reconstruct": (C.c_int, [C.c_void_p, C.c_int, _D, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                   _D, _D]),
     "mhd_check_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
@@ -233,8 +240,12 @@ class Solver:
     def bxyz(self):
         _ck(lib().mhd_get_bxyz(self.h), "mhd_get_bxyz")
 
-    def flux_stress(self):
-        _ck(lib().mhd_flux_stress(self.h), "mhd_flux_stress")
+    def flux_stress(self, which=STATE_CURRENT, compute=True):
+        """get_flux_stress of the current state or (Adams-Bashforth, after
+        predict_ab) the extrapolated one; the 24 face arrays.  compute=False
+        fetches the held arrays."""
+        if compute:
+            _ck(lib().mhd_flux_stress(self.h, int(which)), "mhd_flux_stress")
         g = self.grid
         out = []
         for d in range(3):
@@ -245,8 +256,10 @@ class Solver:
                 out.append(a)
         return out
 
-    def eijk(self):
-        _ck(lib().mhd_eijk_lfm(self.h), "mhd_eijk_lfm")
+    def eijk(self, which=STATE_CURRENT, compute=True):
+        """get_eijk_LFM of the current or the extrapolated state"""
+        if compute:
+            _ck(lib().mhd_eijk_lfm(self.h, int(which)), "mhd_eijk_lfm")
         g = self.grid
         shp = {FETCH_EI: (g.nx0, g.ny0 + 1, g.nz0 + 1), FETCH_EJ: (g.nx0 + 1, g.ny0, g.nz0 + 1),
                FETCH_EK: (g.nx0 + 1, g.ny0 + 1, g.nz0)}
@@ -256,6 +269,34 @@ class Solver:
             _ck(lib().mhd_fetch(self.h, w, _p(a)), "mhd_fetch")
             out[name] = a
         return out
+
+    def set_tau(self, tau, tau0):
+        """the module variables tau and tau0 the update routines use"""
+        _ck(lib().mhd_set_tau(self.h, float(tau), float(tau0)), "mhd_set_tau")
+
+    def get_tau(self):
+        t, t0 = C.c_double(0.), C.c_double(0.)
+        _ck(lib().mhd_get_tau(self.h, C.byref(t), C.byref(t0)), "mhd_get_tau")
+        return t.value, t0.value
+
+    def predict_ab(self):
+        """advance_predictor_ab (+ advance_previous, tau0 = tau)"""
+        _ck(lib().mhd_predict_ab(self.h), "mhd_predict_ab")
+
+    def advance(self, stage):
+        """one stage's update from the held fluxes and EMFs: stage 0 the
+        Adams-Bashforth corrector, 1..stepop a Runge-Kutta stage"""
+        _ck(lib().mhd_advance(self.h, int(stage)), "mhd_advance")
+
+    def store_ops(self, ops):
+        """write face fluxes (ops['flux'], 24 arrays) and edge EMFs
+        (ops['ei'/'ej'/'ek']) for the next advance"""
+        for q, f in enumerate(ops["flux"]):
+            a = np.asfortranarray(f, dtype=np.float64)
+            _ck(lib().mhd_store(self.h, q, _p(a)), "mhd_store")
+        for w, name in ((FETCH_EI, "ei"), (FETCH_EJ, "ej"), (FETCH_EK, "ek")):
+            a = np.asfortranarray(ops[name], dtype=np.float64)
+            _ck(lib().mhd_store(self.h, w, _p(a)), "mhd_store")
 
     def divb_max(self):
         """max |div B| over the owned cells (mhd_divb_max)"""
@@ -336,6 +377,18 @@ def reconstruct(solver, f, lshape, direction):
     _ck(lib().mhd_reconstruct(solver.h, direction, _p(f), f.shape[0], f.shape[1], f.shape[2], lshape[0], lshape[1],
                               lshape[2], _p(fl), _p(fr)), "mhd_reconstruct")
     return fl, fr
+
+
+def v_face(solver, lam, v):
+    """get_v_face through the library in the solver's flux arithmetic:
+    (v0p, v0n, v1p, v1n)"""
+    lam = np.ascontiguousarray(lam, dtype=np.float64).ravel()
+    v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+    if lam.shape != v.shape:
+        raise ValueError("lam and v differ in size")
+    outs = [np.zeros_like(lam) for _ in range(4)]
+    _ck(lib().mhd_v_face(solver.h, lam.size, _p(lam), _p(v), *[_p(o) for o in outs]), "mhd_v_face")
+    return outs
 
 
 def metric_nodes_host(grid):

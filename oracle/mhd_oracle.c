@@ -701,16 +701,12 @@ static inline double prs(double gam, double eng, double mx, double my, double mz
     return (gam - 1.) * (eng - .5 * (sq(mx) + sq(my) + sq(mz)) / r);
 }
 
-/* TVD RK2 / RK3 (mhd3d3.f90:3648-3792); nst = number of stages */
-static double step_rk(orc_state *s, int nst) {
+/* one TVD RK stage's update after the flux (mhd3d3.f90:3669-3707 for RK2,
+ * 3737-3785 for RK3): corrector1, stage combine and pressure, corrector2,
+ * combine and velocity, advance_bijk and the face combine */
+static void rk_update(orc_state *s, int nst, int st) {
     const double gam = s->p.gam;
-    double tcfl = orc_get_dt(s);
-    s->tau = s->tau0;
-    advance_previous(s);
-    for (int st = 1; st <= nst; ++st) {
-        get_conserve(s);
-        eijk_lfm(s, &s->rho, &s->vx, &s->vy, &s->vz, &s->bx, &s->by, &s->bz, &s->bi, &s->bj, &s->bk);
-        flux_stress(s, &s->rho, &s->vx, &s->vy, &s->vz, &s->pr, &s->bx, &s->by, &s->bz);
+    {
         corrector1(s);
         FOR_INTERIOR(s, {
             if (st == 1) {
@@ -766,17 +762,27 @@ static double step_rk(orc_state *s, int nst) {
                 }
             }
         }
+    }
+}
+
+/* TVD RK2 / RK3 (mhd3d3.f90:3648-3792); nst = number of stages */
+static double step_rk(orc_state *s, int nst) {
+    double tcfl = orc_get_dt(s);
+    s->tau = s->tau0;
+    advance_previous(s);
+    for (int st = 1; st <= nst; ++st) {
+        get_conserve(s);
+        eijk_lfm(s, &s->rho, &s->vx, &s->vy, &s->vz, &s->bx, &s->by, &s->bz, &s->bi, &s->bj, &s->bk);
+        flux_stress(s, &s->rho, &s->vx, &s->vy, &s->vz, &s->pr, &s->bx, &s->by, &s->bz);
+        rk_update(s, nst, st);
         orc_fill(s);
         orc_bxyz(s);
     }
     return tcfl;
 }
 
-/* Adams-Bashforth predictor-corrector (mhd3d3.f90:3602-3644, 3821-3840) */
-static void step_ab(orc_state *s) {
-    const double gam = s->p.gam;
-    orc_get_dt(s);
-    get_conserve(s);
+/* advance_predictor_ab (mhd3d3.f90:3821-3840) and the tau0=tau of step_ab (3616) */
+static void ab_predict(orc_state *s) {
     const double c = s->tau / s->tau0 / 2.;
     A3 *x[11] = {&s->rho, &s->vx, &s->vy, &s->vz, &s->pr, &s->bx, &s->by, &s->bz, &s->bi, &s->bj, &s->bk};
     A3 *xp[11] = {&s->rhop, &s->vxp, &s->vyp, &s->vzp, &s->pp, &s->bxp, &s->byp, &s->bzp, &s->bip, &s->bjp, &s->bkp};
@@ -787,8 +793,12 @@ static void step_ab(orc_state *s) {
     }
     advance_previous(s);
     s->tau0 = s->tau;
-    eijk_lfm(s, &s->rhoh, &s->vxh, &s->vyh, &s->vzh, &s->bxh, &s->byh, &s->bzh, &s->bih, &s->bjh, &s->bkh);
-    flux_stress(s, &s->rhoh, &s->vxh, &s->vyh, &s->vzh, &s->ph, &s->bxh, &s->byh, &s->bzh);
+}
+
+/* the Adams-Bashforth update after the flux (mhd3d3.f90:3622-3627):
+ * corrector1 with the pressure, corrector2 with the velocity, advance_bijk */
+static void ab_update(orc_state *s) {
+    const double gam = s->p.gam;
     corrector1(s);
     FOR_INTERIOR(s, { s->pr.d[q] = prs(gam, s->eng.d[q], s->rhovx.d[q], s->rhovy.d[q], s->rhovz.d[q], s->rho.d[q]); });
     corrector2(s);
@@ -798,6 +808,16 @@ static void step_ab(orc_state *s) {
         s->vz.d[q] = s->rhovz.d[q] / s->rho.d[q];
     });
     advance_bijk(s);
+}
+
+/* Adams-Bashforth predictor-corrector (mhd3d3.f90:3602-3644) */
+static void step_ab(orc_state *s) {
+    orc_get_dt(s);
+    get_conserve(s);
+    ab_predict(s);
+    eijk_lfm(s, &s->rhoh, &s->vxh, &s->vyh, &s->vzh, &s->bxh, &s->byh, &s->bzh, &s->bih, &s->bjh, &s->bkh);
+    flux_stress(s, &s->rhoh, &s->vxh, &s->vyh, &s->vzh, &s->ph, &s->bxh, &s->byh, &s->bzh);
+    ab_update(s);
     orc_fill(s);
     orc_bxyz(s);
 }
@@ -940,6 +960,63 @@ void orc_flux_stress(orc_state *s) {
 
 void orc_eijk(orc_state *s) {
     eijk_lfm(s, &s->rho, &s->vx, &s->vy, &s->vz, &s->bx, &s->by, &s->bz, &s->bi, &s->bj, &s->bk);
+}
+
+void orc_set_tau(orc_state *s, double tau, double tau0) {
+    s->tau = tau;
+    s->tau0 = tau0;
+}
+
+void orc_get_tau(const orc_state *s, double *tau, double *tau0) {
+    if (tau) *tau = s->tau;
+    if (tau0) *tau0 = s->tau0;
+}
+
+int orc_predict_ab(orc_state *s) {
+    if (s->p.stepop != 1) return 1;
+    ab_predict(s);
+    return 0;
+}
+
+void orc_flux_stress_state(orc_state *s, int which) {
+    if (which == 1)
+        flux_stress(s, &s->rhoh, &s->vxh, &s->vyh, &s->vzh, &s->ph, &s->bxh, &s->byh, &s->bzh);
+    else
+        orc_flux_stress(s);
+}
+
+void orc_eijk_state(orc_state *s, int which) {
+    if (which == 1)
+        eijk_lfm(s, &s->rhoh, &s->vxh, &s->vyh, &s->vzh, &s->bxh, &s->byh, &s->bzh, &s->bih, &s->bjh, &s->bkh);
+    else
+        orc_eijk(s);
+}
+
+/* get_conserve and the stage update from the held fluxes and EMFs: stage 0
+ * the Adams-Bashforth corrector, 1..nst a TVD RK stage (stage 1 after the
+ * advance_previous that step_rk2 takes just before it, mhd3d3.f90:3664) */
+int orc_advance(orc_state *s, int stage) {
+    const int nst = s->p.stepop == 1 ? 0 : s->p.stepop;
+    if (nst == 0 ? stage != 0 : (stage < 1 || stage > nst)) return 1;
+    if (stage == 1) advance_previous(s);
+    get_conserve(s);
+    if (nst == 0) ab_update(s);
+    else rk_update(s, nst, stage);
+    return 0;
+}
+
+void orc_set_ops(orc_state *s, const double *ei, const double *ej, const double *ek, const double *flux_all) {
+    if (ei) memcpy(s->ei.d, ei, a3_size(&s->ei) * sizeof(double));
+    if (ej) memcpy(s->ej.d, ej, a3_size(&s->ej) * sizeof(double));
+    if (ek) memcpy(s->ek.d, ek, a3_size(&s->ek) * sizeof(double));
+    if (flux_all) {
+        size_t off = 0;
+        for (int q = 0; q < ORC_NFLUX; ++q) {
+            size_t n = a3_size(&s->flux[q]);
+            memcpy(s->flux[q].d, flux_all + off, n * sizeof(double));
+            off += n;
+        }
+    }
 }
 
 void orc_get_ops(const orc_state *s, double *ei, double *ej, double *ek, double *flux_all) {

@@ -48,6 +48,20 @@ double orc_get_dt(orc_state *s);             /* get_dt (min over the ghost-inclu
 void orc_flux_stress(orc_state *s);          /* get_flux_stress on the current state */
 void orc_eijk(orc_state *s);                 /* get_eijk_LFM on the current state */
 void orc_get_ops(const orc_state *s, double *ei, double *ej, double *ek, double *flux_all);
+/* per-routine pieces of the time stepper (mhd3d3.f90:3602-3933):
+ * the module variables tau/tau0; advance_predictor_ab with its
+ * advance_previous and tau0=tau (returns 1 unless stepop is 1); the flux and
+ * EMF of the current (0) or extrapolated (1) state; get_conserve plus the
+ * update of stage 0 (Adams-Bashforth) or 1..stepop (RK; stage 1 first takes
+ * advance_previous) from the held fluxes and EMFs (returns 1 for a stage the
+ * stepper lacks); and writing those fluxes and EMFs (orc_get_ops layout). */
+void orc_set_tau(orc_state *s, double tau, double tau0);
+void orc_get_tau(const orc_state *s, double *tau, double *tau0);
+int orc_predict_ab(orc_state *s);
+void orc_flux_stress_state(orc_state *s, int which);
+void orc_eijk_state(orc_state *s, int which);
+int orc_advance(orc_state *s, int stage);
+void orc_set_ops(orc_state *s, const double *ei, const double *ej, const double *ek, const double *flux_all);
 /* nsteps calls of `step` (mhd3d3.f90:3585); *tau in/out as the deck tau / tau0. */
 int orc_step(orc_state *s, int nsteps, double *tau, double *tau_cfl);
 

@@ -53,6 +53,15 @@ def lib():
         L.orc_flux_stress.argtypes = [C.c_void_p]
         L.orc_eijk.argtypes = [C.c_void_p]
         L.orc_get_ops.argtypes = [C.c_void_p, _D, _D, _D, _D]
+        L.orc_set_tau.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.orc_get_tau.argtypes = [C.c_void_p, _D, _D]
+        L.orc_predict_ab.argtypes = [C.c_void_p]
+        L.orc_predict_ab.restype = C.c_int
+        L.orc_flux_stress_state.argtypes = [C.c_void_p, C.c_int]
+        L.orc_eijk_state.argtypes = [C.c_void_p, C.c_int]
+        L.orc_advance.argtypes = [C.c_void_p, C.c_int]
+        L.orc_advance.restype = C.c_int
+        L.orc_set_ops.argtypes = [C.c_void_p, _D, _D, _D, _D]
         L.orc_step.argtypes = [C.c_void_p, C.c_int, _D, _D]
         L.orc_step.restype = C.c_int
         L.orc_reconstruct.argtypes = [C.POINTER(Params), _D, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -132,10 +141,35 @@ class Oracle:
     def get_dt(self):
         return lib().orc_get_dt(self.h)
 
-    def ops(self):
-        """run get_eijk_LFM + get_flux_stress on the current state."""
-        lib().orc_eijk(self.h)
-        lib().orc_flux_stress(self.h)
+    def set_tau(self, tau, tau0):
+        lib().orc_set_tau(self.h, float(tau), float(tau0))
+
+    def get_tau(self):
+        t, t0 = C.c_double(0.), C.c_double(0.)
+        lib().orc_get_tau(self.h, C.byref(t), C.byref(t0))
+        return t.value, t0.value
+
+    def predict_ab(self):
+        if lib().orc_predict_ab(self.h):
+            raise RuntimeError("orc_predict_ab: not an Adams-Bashforth oracle")
+
+    def advance(self, stage):
+        if lib().orc_advance(self.h, int(stage)):
+            raise RuntimeError("orc_advance: stage %d not in this stepper" % stage)
+
+    def set_ops(self, ops):
+        """store face fluxes and edge EMFs (the layout of ops())"""
+        flat = np.concatenate([np.asarray(f, dtype=np.float64).ravel(order="F") for f in ops["flux"]])
+        arrs = [np.asfortranarray(ops[k], dtype=np.float64) for k in ("ei", "ej", "ek")]
+        lib().orc_set_ops(self.h, *[_p(a) for a in arrs], _p(flat))
+
+    def ops(self, which=0, compute=True):
+        """run get_eijk_LFM + get_flux_stress on the current state (which=0)
+        or the Adams-Bashforth extrapolation (which=1) and return the
+        products; compute=False returns the held products as they are."""
+        if compute:
+            lib().orc_eijk_state(self.h, int(which))
+            lib().orc_flux_stress_state(self.h, int(which))
         g = self.g
         n0 = (g.nx0, g.ny0, g.nz0)
         ei = np.zeros((n0[0], n0[1] + 1, n0[2] + 1), order="F")

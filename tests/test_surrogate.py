@@ -187,9 +187,13 @@ def test_mfma_inference_matches_cpu(ctx):
 
 
 @pytest.mark.gpu
-def test_gpu_learned_flux_matches_oracle():
+@pytest.mark.parametrize("kind", ["orszag_tang", "blobs"])
+def test_gpu_learned_flux_matches_oracle(kind):
+    """the configuration-3 set-ups (the Orszag-Tang vortex of configuration 0
+    and the Gaussian blobs of configuration 2) at a size the oracle finishes
+    in seconds"""
     g = ics.Grid(20, 18, 16, ng=4)
-    st = ics.initial_state(g, "orszag_tang", 33)
+    st = ics.initial_state(g, kind, 33)
     w = _weights(9, 1.)
     s = gpu.Solver(gpu.make_params(20, 18, 16, ng=4, reconop=1, stepop=1, bc="even", precision="mixed",
                                    surrogate=1), g)
@@ -211,7 +215,11 @@ def test_gpu_learned_flux_matches_oracle():
         assert err.max() <= 5e-2, (q, err.max())
         moved += int((np.abs(ref[q] - plain[q]) > 1e-3 * scale).sum())
     # and the net did take part: its fluxes differ from PDM's on some faces
-    assert moved > 0 and 0 < fb < faces
+    # (the resting blobs have no rough face, so there nothing falls back)
+    print("%s: %d of %d faces on the PDM fallback, %d flux values moved by the net" % (kind, fb, faces, moved))
+    assert moved > 0 and fb < faces
+    if kind == "orszag_tang":
+        assert fb > 0
 
 
 TRAIN_THETA = .5
