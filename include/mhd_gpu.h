@@ -191,7 +191,10 @@ int mhd_predict_ab(mhd_ctx *ctx);
  *            advance_previous copies (3794-3819) of the state it starts
  *            from (step_rk2 takes them just before stage 1, 3664).
  * Cell B and the CFL partials follow at mhd_get_bxyz, as in the
- * reference.  MHD_ERR_PARAM for a stage the context's stepper lacks. */
+ * reference.  Only interior cells and owned faces are written: the
+ * reference's whole-array stage combine also averages the ghost layers,
+ * which the ghost fill that follows overwrites in both.  MHD_ERR_PARAM for a
+ * stage the context's stepper lacks. */
 int mhd_advance(mhd_ctx *ctx, int stage);
 
 /* copy a device product to the host:

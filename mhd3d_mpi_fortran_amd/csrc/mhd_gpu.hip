@@ -321,7 +321,8 @@ int mhd_set_metric(mhd_ctx* c, const double* A) {
 // column-major, node (i,j,k) = lower corner of cell (i,j,k), ghost cells
 // included).  Face area vectors S = (d1 x d2)/2 of the face diagonals
 // (oriented along increasing index), volumes by the divergence theorem with
-// face centroids, cell B coefficients from B_c = (1/V) sum_f (B.S_out)(x_f - x_c),
+// face centroids, cell B coefficients of the area-weighted least-squares fit
+// of B to the six face normal components,
 // and for each owned edge the factor |S1||S2|/V of its local basis (edge
 // vector dl and the averaged transverse node differences a1, a2), which
 // turns the face-normal velocities and fields into E . dl exactly for a
@@ -401,26 +402,43 @@ long mhd_metric_nodes_host(int nx0, int ny0, int nz0, int ng, const double* X, c
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) {
                 const long t = (long)(i - 1) + (long)nx * ((long)(j - 1) + (long)ny * (k - 1));
-                double S[3][2][3], xf[3][2][3], xc[3] = {0., 0., 0.}, p[3];
+                double S[3][2][3], xf[3][2][3];
                 for (int f = 0; f < 3; ++f) {
                     face(f, i, j, k, S[f][0], xf[f][0]);
                     face(f, i + (f == 0), j + (f == 1), k + (f == 2), S[f][1], xf[f][1]);
-                }
-                for (int c8 = 0; c8 < 8; ++c8) {
-                    node(i + (c8 & 1), j + ((c8 >> 1) & 1), k + ((c8 >> 2) & 1), p);
-                    for (int r = 0; r < 3; ++r) xc[r] += .125 * p[r];
                 }
                 double V = 0.;
                 for (int f = 0; f < 3; ++f) V += dot(S[f][1], xf[f][1]) - dot(S[f][0], xf[f][0]);
                 V /= 3.;
                 if (!(V > 0.)) return -MHD_ERR_PARAM;
                 ivol[t] = 1. / V;
+                // cell B from the six face normal components: the area-weighted
+                // least-squares fit B = N^-1 sum_f |S_f| n_f beta_f, N = sum_f
+                // |S_f| n_f n_f^T -- exact for a uniform field on any grid (a
+                // curved cell's faces are not planar, so the divergence-theorem
+                // form (1/V) sum_f (B.S_f)(x_f - x_c) is exact on affine cells
+                // only), and on affine cells the same map as that form
+                double N[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}}, un[6][3], wf[6];
+                for (int f = 0; f < 6; ++f) {
+                    const double* Sv = S[f / 2][f % 2];
+                    wf[f] = norm(Sv);
+                    for (int r = 0; r < 3; ++r) un[f][r] = Sv[r] / wf[f];
+                    for (int r = 0; r < 3; ++r)
+                        for (int q = 0; q < 3; ++q) N[r][q] += wf[f] * un[f][r] * un[f][q];
+                }
+                double Ni[3][3];
+                for (int r = 0; r < 3; ++r)
+                    for (int q = 0; q < 3; ++q)
+                        Ni[q][r] = N[(r + 1) % 3][(q + 1) % 3] * N[(r + 2) % 3][(q + 2) % 3] -
+                                   N[(r + 1) % 3][(q + 2) % 3] * N[(r + 2) % 3][(q + 1) % 3];
+                const double detN = N[0][0] * Ni[0][0] + N[0][1] * Ni[1][0] + N[0][2] * Ni[2][0];
+                if (!(detN > 0.)) return -MHD_ERR_PARAM;
+                for (int f = 0; f < 6; ++f)
+                    for (int r = 0; r < 3; ++r)
+                        cm[(f * 3 + r) * ncell + t] =
+                            wf[f] * (Ni[r][0] * un[f][0] + Ni[r][1] * un[f][1] + Ni[r][2] * un[f][2]) / detN;
                 for (int f = 0; f < 3; ++f) {
                     const double a0 = norm(S[f][0]), a1 = norm(S[f][1]);
-                    for (int r = 0; r < 3; ++r) {
-                        cm[((f * 2 + 0) * 3 + r) * ncell + t] = -a0 * (xf[f][0][r] - xc[r]) / V;
-                        cm[((f * 2 + 1) * 3 + r) * ncell + t] = a1 * (xf[f][1][r] - xc[r]) / V;
-                    }
                     double sn[3];
                     for (int r = 0; r < 3; ++r) sn[r] = S[f][0][r] + S[f][1][r];
                     const double ln = norm(sn);

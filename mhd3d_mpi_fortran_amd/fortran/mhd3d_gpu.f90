@@ -26,8 +26,11 @@
 !       icfile             initial-state file (global owned arrays, float64,
 !                          column-major: rho vx vy vz p (nx0G,ny0G,nz0G),
 !                          bi (nx0G+1,..), bj, bk); empty = shock tube
-!       outdir             frame files frame_<n>.bin are written there
-!                          (empty = no output)
+!       outdir             output directory (empty = no output): the
+!                          reference's outputxyz datasets and stime.out
+!                          (see write_xyz) and, with
+!       frames             .true., also one raw file frame_<n>.bin per frame
+!                          with the face fields (see write_frame)
 !-----------------------------------------------------------------------
 program mhd3d_gpu
    use, intrinsic :: iso_c_binding
@@ -45,13 +48,14 @@ program mhd3d_gpu
    integer :: surrogate, device
    character(len=512) :: icfile, outdir, weights
    real(8) :: theta, metric(9)
+   logical :: frames
 
    namelist /dom/ xmin, xmax, ymin, ymax, zmin, zmax
    namelist /grid/ nx0G, ny0G, nz0G, xprocs, yprocs, zprocs
    namelist /comset/ nord, reconop, stepop, outop
    namelist /list/ nts, tau, nframes, nrst, nfdump
    namelist /free/ cfl, pdmb, eps, alpha, beta
-   namelist /gpu/ bc_x, bc_y, bc_z, precision, surrogate, device, icfile, outdir, weights, theta, metric
+   namelist /gpu/ bc_x, bc_y, bc_z, precision, surrogate, device, icfile, outdir, weights, theta, metric, frames
 
    real(8), allocatable :: rho(:,:,:), vx(:,:,:), vy(:,:,:), vz(:,:,:), p(:,:,:)
    real(8), allocatable :: bx(:,:,:), by(:,:,:), bz(:,:,:)
@@ -72,6 +76,7 @@ program mhd3d_gpu
    bc_x = 'even'; bc_y = 'even'; bc_z = 'even'; precision = 'fp64'
    surrogate = 0; device = 0; icfile = ''; outdir = ''; weights = ''; theta = 0.25d0
    metric = [1.d0, 0.d0, 0.d0, 0.d0, 1.d0, 0.d0, 0.d0, 0.d0, 1.d0]
+   frames = .false.
    xprocs = 1; yprocs = 1; zprocs = 1; outop = 0; nrst = 0; nfdump = 0
 
    print*, 'reading input file ', trim(deck)
@@ -130,7 +135,7 @@ program mhd3d_gpu
    print*, 'initialization finished!'
 
    time = 0.
-   if (len_trim(outdir) > 0) call write_frame(0)
+   if (len_trim(outdir) > 0) call output(0)
 
    call system_clock(c0, crate)
    do ntt = 1, nframes
@@ -153,7 +158,7 @@ program mhd3d_gpu
          stop 'Non-positive pressure found. Simulation terminated!'
       endif
       call mhd_check(rc, 'mhd_check_state')
-      if (len_trim(outdir) > 0) call write_frame(ntt)
+      if (len_trim(outdir) > 0) call output(ntt)
       print '(A,I0,A,ES14.6)', ' output frame number ', ntt+1, ' time=', time
    enddo
    call mhd_check(mhd_synchronize(ctx), 'mhd_synchronize')
@@ -250,6 +255,59 @@ contains
       close(v)
    end subroutine read_ic
 
+   ! frame output after frame n: the state comes back from the device once
+   subroutine output(n)
+      integer, intent(in) :: n
+      call mhd_check(mhd_download_state(ctx, rho, vx, vy, vz, p, bi, bj, bk, bx, by, bz), &
+                     'mhd_download_state')
+      call write_xyz(n)
+      if (frames) call write_frame(n)
+   end subroutine output
+
+   ! outputxyz (mhd3d3.f90:3091-3109) into the datasets createh5_3D makes
+   ! (2966-3000): per variable rho vx vy vz p bx by bz one dataset of shape
+   ! (nx0G, ny0G, nz0G, nframes+1), float64, column-major, frame n the n-th
+   ! 3-D slab -- here a file <outdir>/<var>.xyz holding the dataset's bytes
+   ! (little-endian, no header: the image has no HDF5 library).  Plus
+   ! output_simt (3112-3127): stime.out, Fortran sequential unformatted, one
+   ! record holding `time` per frame, frame 0 creating the file.
+   subroutine write_xyz(n)
+      integer, intent(in) :: n
+      character(len=3), parameter :: names(8) = ['rho', 'vx ', 'vy ', 'vz ', 'p  ', 'bx ', 'by ', 'bz ']
+      character(len=600) :: fname
+      integer :: v, q, i0, i1, j0, j1, k0, k1
+      integer(8) :: pos
+      i0 = ng + 1; i1 = ng + nx0G; j0 = ng + 1; j1 = ng + ny0G; k0 = ng + 1; k1 = ng + nz0G
+      pos = int(n, 8) * 8_8 * int(nx0G, 8) * int(ny0G, 8) * int(nz0G, 8) + 1_8
+      do q = 1, 8
+         fname = trim(outdir)//'/'//trim(names(q))//'.xyz'
+         if (n == 0) then
+            open(newunit=v, file=trim(fname), access='stream', form='unformatted', status='replace')
+         else
+            open(newunit=v, file=trim(fname), access='stream', form='unformatted', status='old')
+         endif
+         select case (q)
+         case (1); write(v, pos=pos) rho(i0:i1, j0:j1, k0:k1)
+         case (2); write(v, pos=pos) vx(i0:i1, j0:j1, k0:k1)
+         case (3); write(v, pos=pos) vy(i0:i1, j0:j1, k0:k1)
+         case (4); write(v, pos=pos) vz(i0:i1, j0:j1, k0:k1)
+         case (5); write(v, pos=pos) p(i0:i1, j0:j1, k0:k1)
+         case (6); write(v, pos=pos) bx(i0:i1, j0:j1, k0:k1)
+         case (7); write(v, pos=pos) by(i0:i1, j0:j1, k0:k1)
+         case (8); write(v, pos=pos) bz(i0:i1, j0:j1, k0:k1)
+         end select
+         close(v)
+      enddo
+      fname = trim(outdir)//'/stime.out'
+      if (n == 0) then
+         open(newunit=v, file=trim(fname), status='replace', form='unformatted')
+      else
+         open(newunit=v, file=trim(fname), status='old', form='unformatted', position='append')
+      endif
+      write(v) time
+      close(v)
+   end subroutine write_xyz
+
    ! owned state of one frame (stream, little-endian): six int32 (block
    ! coordinates 0 0 0, nx0G ny0G nz0G), time, tau, then rho vx vy vz p bi
    ! bj bk without ghosts -- the per-frame variables of outputxyz
@@ -258,8 +316,6 @@ contains
       integer, intent(in) :: n
       character(len=600) :: fname
       integer :: v
-      call mhd_check(mhd_download_state(ctx, rho, vx, vy, vz, p, bi, bj, bk, bx, by, bz), &
-                     'mhd_download_state')
       write(fname, '(A,A,I4.4,A)') trim(outdir), '/frame_', n, '.bin'
       open(newunit=v, file=trim(fname), access='stream', form='unformatted', status='replace')
       write(v) 0, 0, 0, nx0G, ny0G, nz0G

@@ -225,8 +225,9 @@ def node_metric(grid):
     """the per-cell and per-edge factors of the general metric (numpy, same
     definitions as the library's host builder):
       cm (18, cells): cell B = sum over the 6 faces (i lo, i hi, j lo, ...) of
-         beta_f * cm[3 f + r], from B_c = (1/V) sum_f (B.S_out)(x_f - x_c),
-         x_c the mean of the 8 corners;
+         beta_f * cm[3 f + r]: the area-weighted least-squares fit
+         B = N^-1 sum_f |S_f| n_f beta_f, N = sum_f |S_f| n_f n_f^T, exact for
+         a uniform field on any grid;
       nc (9, cells): unit vector of S_lo + S_hi per family (cell normals for
          the face-normal velocities);
       ih (3, cells): (|S_lo| + |S_hi|) / (2 V), the inverse spacing of the CFL;
@@ -239,10 +240,9 @@ def node_metric(grid):
     fv = face_vectors(X, Y, Z)
     geo = grid.geometry()
     vol = geo["vol"]
-    xc = np.stack([_corner_mean(a) for a in (X, Y, Z)])
-    cm = []
     nc = []
     ih = []
+    wn = []    # |S_f| n_f of the 6 faces (i lo, i hi, j lo, ...)
     for f, (S, xf) in enumerate(fv):
         lo = [slice(None)] * 4
         hi = [slice(None)] * 4
@@ -251,11 +251,16 @@ def node_metric(grid):
         S0, S1 = S[tuple(lo)], S[tuple(hi)]
         a0 = np.sqrt((S0 * S0).sum(axis=0))
         a1 = np.sqrt((S1 * S1).sum(axis=0))
-        cm.extend(list(-a0 * (xf[tuple(lo)] - xc) / vol))
-        cm.extend(list(a1 * (xf[tuple(hi)] - xc) / vol))
+        wn.extend([S0, S1])
         sn = S0 + S1
         nc.extend(list(sn / np.sqrt((sn * sn).sum(axis=0))))
         ih.append(.5 * (a0 + a1) / vol)
+    # N = sum_f |S_f| n_f n_f^T = sum_f S_f S_f^T / |S_f|
+    N = sum(np.einsum("r...,q...->rq...", w, w) / np.sqrt((w * w).sum(axis=0)) for w in wn)
+    Ni = np.moveaxis(np.linalg.inv(np.moveaxis(N, (0, 1), (-2, -1))), (-2, -1), (0, 1))
+    cm = []
+    for w in wn:
+        cm.extend(list(np.einsum("rq...,q...->r...", Ni, w)))
     ng = grid.ng
     ef = []
     for e in range(3):

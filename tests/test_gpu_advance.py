@@ -54,6 +54,16 @@ def same(a, b, keys):
         assert np.array_equal(a[k], b[k]), k
 
 
+def same_owned(g, a, b, keys):
+    """interior cells and owned faces: between an update and the ghost fill
+    the ghost layers hold different leftovers (the reference's stage combine
+    averages whole arrays, ghosts included; the library combines the owned
+    values), which the fill then overwrites on both sides"""
+    for k in keys:
+        f = {"bi": 1, "bj": 2, "bk": 3}.get(k, 0)
+        assert np.array_equal(g.interior(a[k], f), g.interior(b[k], f)), k
+
+
 CASES = [(1, 2, "even"), (1, 2, "periodic"), (2, 3, "periodic"), (4, 3, "even"), (1, 1, "periodic"),
          (3, 1, "even")]
 
@@ -91,7 +101,7 @@ def test_advance_stages_bit_exact(reconop, stepop, bc):
             assert np.array_equal(s.flux_stress(compute=False)[5], ref["flux"][5])
             s.advance(stage)
             o.advance(stage)
-            same(s.download(), o.get_state(), PRIMARY)
+            same_owned(g, s.download(), o.get_state(), PRIMARY)
             s.fill()
             o.fill()
             s.bxyz()
@@ -147,7 +157,10 @@ def test_chain_equals_step(precision, reconop, stepop, bc, sheared):
     n = (16, 12, 10)
     A = [[1., .5, 0.], [0., 1., 0.], [0., 0., 1.]] if sheared else None
     g = ics.Grid(*n, ng=4, A=A)
-    st = ics.initial_state(g, "blast", 3, radius=.2, p_in=10., p_out=.1)
+    # (WENO5 is not positivity-preserving on the blast's pressure jump; the
+    # Orszag-Tang vortex is set up on the reference's grid only)
+    st = (ics.initial_state(g, "blast", 3, radius=.2, p_in=10., p_out=.1) if sheared else
+          ics.initial_state(g, "orszag_tang", 3))
     out = []
     for routine in (False, True):
         s = gpu.Solver(gpu.make_params(*n, ng=4, reconop=reconop, stepop=stepop, bc=bc, precision=precision), g)
@@ -162,6 +175,7 @@ def test_chain_equals_step(precision, reconop, stepop, bc, sheared):
         s.close()
     (a, ta), (b, tb) = out
     assert ta == tb
+    assert np.isfinite(a["rho"]).all()
     same(a, b, PRIMARY + ("bx", "by", "bz"))
 
 
@@ -194,7 +208,9 @@ def test_get_dt_after_upload(precision):
     """get_dt right after an upload uses cell B of the uploaded faces"""
     g, st, s, o = pair(1, 2, "periodic", kind="orszag_tang", precision=precision)
     s.initialize(1e-3)
+    full = s.download()             # ghost-filled state
     s.step(2, 1e-3)                 # cell B of another state in the arrays
-    s.upload(st)
+    s.upload(full)
+    o.set_state(full)
     o.bxyz()
     assert s.get_dt() == o.get_dt()

@@ -27,7 +27,7 @@ DECK = """ &dom
  cfl=.3, pdmb=4., eps=1.e-8, alpha=2., beta=4.
 /
  &gpu
- bc_x='{bc}', bc_y='{bc}', bc_z='{bc}', outdir='{out}', icfile='{ic}'
+ bc_x='{bc}', bc_y='{bc}', bc_z='{bc}', outdir='{out}', icfile='{ic}', frames=.true.
 /
 """
 
@@ -95,7 +95,7 @@ DECK_GPU = """ &dom
  cfl=.3, pdmb=4., eps=1.e-8, alpha=2., beta=4.
 /
  &gpu
- bc_x='periodic', bc_y='periodic', bc_z='periodic', outdir='{out}', icfile='{ic}',
+ bc_x='periodic', bc_y='periodic', bc_z='periodic', outdir='{out}', icfile='{ic}', frames=.true.,
  precision='mixed', surrogate=1, weights='{w}', theta=0.25,
  metric=1., 0.5, 0., 0., 1., 0., 0., 0., 1.
 /
@@ -138,3 +138,59 @@ def test_driver_metric_and_learned_reconstruction(tmp_path):
             face = {"bi": 1, "bj": 2, "bk": 3}.get(v, 0)
             assert np.array_equal(got[v], g.interior(ref[v], face)), (fr, v)
     s.close()
+
+
+def read_xyz(path, n, nframes):
+    """one dataset of outputxyz: (nx0G, ny0G, nz0G, nframes+1) float64"""
+    a = np.fromfile(path, "<f8")
+    assert a.size == n[0] * n[1] * n[2] * (nframes + 1), (path, a.size)
+    return a.reshape(tuple(n) + (nframes + 1,), order="F")
+
+
+def read_records(path):
+    """Fortran sequential unformatted records: 4-byte length, data, length"""
+    raw = open(path, "rb").read()
+    out, off = [], 0
+    while off < len(raw):
+        ln = int(np.frombuffer(raw, "<i4", 1, off)[0])
+        assert int(np.frombuffer(raw, "<i4", 1, off + 4 + ln)[0]) == ln
+        out.append(raw[off + 4:off + 4 + ln])
+        off += 8 + ln
+    return out
+
+
+def test_driver_outputxyz_layout(tmp_path):
+    """the reference's outputxyz datasets (rho vx vy vz p bx by bz, shape
+    (nx0G, ny0G, nz0G, nframes+1), mhd3d3.f90:2966-3109) and output_simt's
+    stime.out records (3112-3127): frame n of each dataset equals the raw
+    frame of the same run (cell B from its faces, as get_bxyz forms it), and
+    stime.out holds the frame times"""
+    c = Case("ot_pdm_ab2_per")
+    m = c.meta
+    ic = str(tmp_path / "ic.bin")
+    with open(ic, "wb") as fh:
+        for v in STATE:
+            fh.write(np.asfortranarray(c.z["ic_" + v]).tobytes(order="F"))
+    nframes = max(m["frames"])
+    deck = tmp_path / "mhd3d3.in"
+    deck.write_text(DECK.format(lo=m["lo"], hi=m["hi"], n=m["grid"], nord=2 * m["ng"], reconop=m["reconop"],
+                                stepop=m["stepop"], tau=m["tau"], nframes=nframes, bc=m["bc"], out=str(tmp_path),
+                                ic=ic))
+    r = subprocess.run([build.DRIVER, str(deck)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    n = m["grid"]
+    data = {v: read_xyz(tmp_path / (v + ".xyz"), n, nframes) for v in ("rho", "vx", "vy", "vz", "p", "bx", "by", "bz")}
+    for fr in range(nframes + 1):
+        raw = read_frame(tmp_path / ("frame_%04d.bin" % fr), n)
+        for v in ("rho", "vx", "vy", "vz", "p"):
+            assert np.array_equal(data[v][..., fr], raw[v]), (fr, v)
+        assert np.array_equal(data["bx"][..., fr], .5 * (raw["bi"][1:] + raw["bi"][:-1])), fr
+        assert np.array_equal(data["by"][..., fr], .5 * (raw["bj"][:, 1:] + raw["bj"][:, :-1])), fr
+        assert np.array_equal(data["bz"][..., fr], .5 * (raw["bk"][:, :, 1:] + raw["bk"][:, :, :-1])), fr
+    recs = read_records(tmp_path / "stime.out")
+    times = [float(np.frombuffer(b, "<f8")[0]) for b in recs]
+    assert len(times) == nframes + 1 and times[0] == 0.
+    assert all(b > a for a, b in zip(times, times[1:]))
+    # the driver prints the same frame times
+    printed = [float(line.split("time=")[1]) for line in r.stdout.splitlines() if "output frame number" in line]
+    assert np.allclose(times[1:], printed, rtol=1e-6)

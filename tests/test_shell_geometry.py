@@ -65,3 +65,153 @@ def test_dipole_is_divergence_free():
     assert np.abs(d).max() * hmin / bmax <= 1e-13
     # and it is the dipole: radial face field at the equator, r = 3 face, ~ -m/r^3 (B along -z there)
     assert np.isfinite(st["rho"]).all() and abs(st["rho"].mean() - 1.) < 1e-2
+
+
+# ---------------------------------------------------------------------------
+# the general metric's arrays on curved cells (the library's host builder,
+# mhd_metric_nodes_host; no device needed), against closed forms
+# ---------------------------------------------------------------------------
+def _host_metric(g):
+    from mhd3d_mpi_fortran_amd import gpu
+    return gpu.metric_nodes_host(g)
+
+
+def _cell_b(m, beta):
+    """cell B from the six face normal components through the coefficients"""
+    lohi = [(beta[0][:-1], beta[0][1:]), (beta[1][:, :-1], beta[1][:, 1:]), (beta[2][:, :, :-1], beta[2][:, :, 1:])]
+    out = 0.
+    for f in range(3):
+        for s in range(2):
+            out = out + lohi[f][s][None] * m["cm"][(2 * f + s) * 3:(2 * f + s) * 3 + 3]
+    return out
+
+
+def test_cell_b_of_uniform_field_is_exact_on_the_shell():
+    """a uniform B given by its face normal components comes back exactly
+    from the 18 coefficients of every cell, ghosts included; the
+    divergence-theorem form (1/V) sum_f (B.S_f)(x_f - x_c) -- exact on
+    planar-faced cells -- misses it on the shell's curved cells by ~1 %, so a
+    coefficient set of that kind fails this check"""
+    g = shell.spherical_shell(12, 10, 16, ng=3, theta0=.6)
+    m = _host_metric(g)
+    B = np.array([.3, -.7, .45])
+    beta = [np.einsum("r...,r->...", m["nf"][f], B) for f in range(3)]
+    err = np.abs(_cell_b(m, beta) - B[:, None, None, None]).max()
+    assert err <= 1e-13, err
+    # the numpy mirror builds the same coefficients
+    mn = shell.node_metric(g)
+    assert np.abs(mn["cm"] - m["cm"]).max() <= 1e-12 * np.abs(m["cm"]).max()
+    # planted: the divergence-theorem coefficients
+    X, Y, Z = g.nodes
+    fv = shell.face_vectors(X, Y, Z)
+    vol = g.geometry()["vol"]
+    xc = np.stack([shell._corner_mean(a) for a in (X, Y, Z)])
+    cm_dt = []
+    for f, (S, xf) in enumerate(fv):
+        lo, hi = [slice(None)] * 4, [slice(None)] * 4
+        lo[f + 1], hi[f + 1] = slice(None, -1), slice(1, None)
+        a0 = np.sqrt((S[tuple(lo)] ** 2).sum(axis=0))
+        a1 = np.sqrt((S[tuple(hi)] ** 2).sum(axis=0))
+        cm_dt.extend(list(-a0 * (xf[tuple(lo)] - xc) / vol))
+        cm_dt.extend(list(a1 * (xf[tuple(hi)] - xc) / vol))
+    bad = _cell_b(dict(cm=np.stack(cm_dt)), beta)
+    assert np.abs(bad - B[:, None, None, None]).max() > 1e3 * 1e-13
+
+
+def _edges(g, e):
+    X, Y, Z = g.nodes
+    P = np.stack([X, Y, Z])
+    ng = g.ng
+    n0 = [g.nx0 + (e != 0), g.ny0 + (e != 1), g.nz0 + (e != 2)]
+    s0 = tuple(slice(ng, ng + n0[q]) for q in range(3))
+    s1 = tuple(slice(ng + (q == e), ng + (q == e) + n0[q]) for q in range(3))
+    return P[(slice(None),) + s1] - P[(slice(None),) + s0]
+
+
+def test_edge_factors_closed_form_on_the_shell():
+    """on an orthogonal grid the edge factor |S1||S2|/V_e of an edge's local
+    basis is the edge's length.  The shell's r and theta edges have exactly
+    orthogonal bases (the transverse node differences are symmetric about
+    them), so ef = |dl| to rounding; a phi edge's basis is orthogonal up to
+    the curvature of its transverse differences, so ef/|dl| -> 1 at fourth
+    order.  A one-sided transverse difference (a plausible slip) makes the
+    bases of all three families non-orthogonal at first order."""
+    errs = []
+    for mult in (1, 2):
+        g = shell.spherical_shell(12 * mult, 10 * mult, 16 * mult, ng=3, theta0=.6)
+        m = _host_metric(g)
+        e3 = []
+        for e in range(3):
+            dl = _edges(g, e)
+            L = np.sqrt((dl * dl).sum(axis=0))
+            e3.append(np.abs(m["ef"][e] / L - 1.).max())
+        assert e3[0] <= 1e-13 and e3[1] <= 1e-13, e3
+        errs.append(e3[2])
+    assert errs[0] <= 1e-3 and errs[1] < errs[0] / 12., errs
+    # planted: transverse differences taken one-sided from the edge's first node
+    g = shell.spherical_shell(12, 10, 16, ng=3, theta0=.6)
+    X, Y, Z = g.nodes
+    P = np.stack([X, Y, Z])
+    ng = g.ng
+    e = 0
+    a, b = 1, 2
+    n0 = [g.nx0, g.ny0 + 1, g.nz0 + 1]
+
+    def nodes(sh):
+        return P[(slice(None),) + tuple(slice(ng + sh[q], ng + sh[q] + n0[q]) for q in range(3))]
+
+    dl = nodes([1, 0, 0]) - nodes([0, 0, 0])
+    va = nodes([0, 1, 0]) - nodes([0, 0, 0])
+    vb = nodes([0, 0, 1]) - nodes([0, 0, 0])
+    S1, S2 = np.cross(vb, dl, axis=0), np.cross(dl, va, axis=0)
+    ef_bad = np.sqrt((S1 * S1).sum(0)) * np.sqrt((S2 * S2).sum(0)) / (va * S1).sum(0)
+    L = np.sqrt((dl * dl).sum(axis=0))
+    assert np.abs(ef_bad / L - 1.).max() > 1e-6
+
+
+def test_face_normals_and_areas_closed_form_on_the_shell():
+    """phi faces lie in meridional planes: unit normal = e_phi, area = the
+    planar quadrilateral's (r1^2 - r0^2) sin(dtheta)/2; theta faces lie in the
+    plane through the origin spanned by their two radial generators: normal
+    along e_r(phi0) x e_r(phi1), area (r1^2 - r0^2) sin(gamma)/2 with
+    cos gamma = sin^2 theta cos dphi + cos^2 theta; r faces tend to the
+    spherical patch r^2 (cos theta0 - cos theta1) dphi at second order"""
+    errs = []
+    for mult in (1, 2):
+        g = shell.spherical_shell(12 * mult, 10 * mult, 16 * mult, ng=3, theta0=.6)
+        m = _host_metric(g)
+        X, Y, Z = g.nodes
+        R = np.sqrt(X * X + Y * Y + Z * Z)
+        TH = np.arccos(Z / R)
+        PH = np.arctan2(Y, X)
+        # phi faces (family 2): nodes (i..i+1, j..j+1) at node k
+        r0, r1 = R[:-1, :-1, :], R[1:, :-1, :]
+        th0, th1 = TH[:-1, :-1, :], TH[:-1, 1:, :]
+        ph = PH[:-1, :-1, :]
+        nf = m["nf"][2]
+        assert np.abs(nf[0] + np.sin(ph)).max() <= 1e-13 and np.abs(nf[1] - np.cos(ph)).max() <= 1e-13
+        assert np.abs(nf[2]).max() <= 1e-13
+        area = .5 * (r1 ** 2 - r0 ** 2) * np.sin(th1 - th0)
+        assert np.abs(m["af"][2] / area - 1.).max() <= 1e-12
+        # theta faces (family 1): nodes (i..i+1, j, k..k+1)
+        r0, r1 = R[:-1, :, :-1], R[1:, :, :-1]
+        th = TH[:-1, :, :-1]
+        p0, p1 = PH[:-1, :, :-1], PH[:-1, :, 1:]
+        er0 = np.stack([np.sin(th) * np.cos(p0), np.sin(th) * np.sin(p0), np.cos(th)])
+        er1 = np.stack([np.sin(th) * np.cos(p1), np.sin(th) * np.sin(p1), np.cos(th)])
+        # orientation: increasing theta, so the normal is the z x ... side: e_r(phi0) x e_r(phi1) points to -theta
+        c = np.cross(er1, er0, axis=0)
+        c /= np.sqrt((c * c).sum(axis=0))
+        assert np.abs(m["nf"][1] - c).max() <= 1e-12
+        cosg = np.sin(th) ** 2 * np.cos(p1 - p0) + np.cos(th) ** 2
+        area = .5 * (r1 ** 2 - r0 ** 2) * np.sqrt(1. - cosg ** 2)
+        assert np.abs(m["af"][1] / area - 1.).max() <= 1e-10
+        # r faces (family 0): the spherical patch, owned faces
+        ng = g.ng
+        sl = (slice(ng, -ng), slice(ng, -ng - 1), slice(ng, -ng - 1))
+        r = R[:, :-1, :-1][sl]
+        t0, t1 = TH[:, :-1, :-1][sl], TH[:, 1:, :-1][sl]
+        dph = (2. * np.pi / g.nz0)
+        patch = r ** 2 * (np.cos(t0) - np.cos(t1)) * dph
+        errs.append(np.abs(m["af"][0][sl] / patch - 1.).max())
+    assert errs[1] < errs[0] / 3.5, errs
