@@ -222,6 +222,20 @@ int mhd_reconstruct(mhd_ctx *ctx, int dir, const double *f, int fn1, int fn2, in
 int mhd_v_face(mhd_ctx *ctx, long n, const double *lam, const double *v, double *v0p, double *v0n, double *v1p,
                double *v1n);
 
+/* mhd_reconstruct without any device allocation (no counterpart in the
+ * reference; for callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and errors as mhd_reconstruct, except that f, fl
+ * and fr are staged through one fixed device array per GPU of
+ * MHD_NOALLOC_DOUBLES doubles (32 MiB, held until the process ends):
+ * fn1*fn2*fn3 + 2*lnx*lny*lnz, each rounded up to a multiple of 32, must not
+ * exceed it (MHD_ERR_PARAM otherwise).  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (from several threads or
+ * contexts) race and must be serialised by the caller. */
+enum { MHD_NOALLOC_DOUBLES = 1 << 22 };
+int mhd_reconstruct_noalloc(mhd_ctx *ctx, int dir, const double *f, int fn1, int fn2, int fn3, int lnx, int lny,
+                            int lnz, double *fl, double *fr);
+
 /* divergence monitor (no counterpart in the reference, which has no div B
  * diagnostic): *divb = max over the owned cells of |div B| of the current
  * face field, each cell's net face flux over its volume as the update forms

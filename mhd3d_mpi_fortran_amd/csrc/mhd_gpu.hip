@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -55,6 +56,30 @@ const char* mhd_error_string(int code) {
 
 static int min_ghosts(int reconop) { return reconop == 1 ? 4 : (reconop == 2 ? 2 : 3); }
 
+// ---------------------------------------------------------------------------
+// scratch of the entry point that allocates nothing (mhd_reconstruct_noalloc):
+// one file-scope device array per GPU, its address looked up once per device
+// with that device current.  Host code uses only the looked-up address: the
+// array's own name is a host-side stand-in, not a device address.
+// ---------------------------------------------------------------------------
+constexpr long kScratchDoubles = 1L << 22;   // 32 MiB per device
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kScratchDevices] = {};
+
+// with `device` current: the device address of this GPU's scratch
+static double* scratch_on(int device) {
+    if (device < 0 || device >= kScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return nullptr;
+        g_scratch_ptr[device] = static_cast<double*>(ptr);
+    }
+    return g_scratch_ptr[device];
+}
+
 int mhd_create(mhd_ctx** out, const mhd_params* p) {
     if (!out || !p) return MHD_ERR_PARAM;
     *out = nullptr;
@@ -73,6 +98,9 @@ int mhd_create(mhd_ctx** out, const mhd_params* p) {
     if (p->precision != MHD_FP64 && p->precision != MHD_MIXED) return MHD_ERR_PARAM;
     if (p->surrogate && (p->reconop != 1 || p->precision != MHD_MIXED)) return MHD_ERR_UNSUPPORTED;
     if (hipSetDevice(p->device) != hipSuccess) return MHD_ERR_HIP;
+    // set-up of the no-allocation entry point's scratch on this device (a
+    // failure here surfaces at that entry point, which retries)
+    (void)scratch_on(p->device);
 
     mhd_ctx* c = new mhd_ctx();
     c->p = *p;
@@ -1288,12 +1316,12 @@ int mhd_fetch(mhd_ctx* c, int which, double* out) {
     return sync_status(c);
 }
 
-int mhd_reconstruct(mhd_ctx* c, int dir, const double* f, int fn1, int fn2, int fn3, int lnx, int lny, int lnz,
-                    double* fl, double* fr) {
+// arguments of reconstruct_3d: every stencil tap must stay inside f
+static int recon_args(const mhd_ctx* c, int dir, const double* f, int fn1, int fn2, int fn3, int lnx, int lny,
+                      int lnz, const double* fl, const double* fr) {
     if (!c || !f || !fl || !fr || dir < 1 || dir > 6 || fn1 <= 0 || fn2 <= 0 || fn3 <= 0 || lnx <= 0 ||
         lny <= 0 || lnz <= 0)
         return MHD_ERR_PARAM;
-    // every stencil tap must stay inside f
     const int ng = c->p.ng;
     const int lo = c->p.reconop == 1 ? -4 : (c->p.reconop == 2 ? -2 : -3);
     const int hi = c->p.reconop == 1 ? 3 : (c->p.reconop == 2 ? 1 : 2);
@@ -1304,6 +1332,12 @@ int mhd_reconstruct(mhd_ctx* c, int dir, const double* f, int fn1, int fn2, int 
         const int mn = 1 + off + (a == ax - 1 ? lo : 0), mx = l[a] + off + (a == ax - 1 ? hi : 0);
         if (mn < 1 || mx > fn[a]) return MHD_ERR_PARAM;
     }
+    return MHD_OK;
+}
+
+int mhd_reconstruct(mhd_ctx* c, int dir, const double* f, int fn1, int fn2, int fn3, int lnx, int lny, int lnz,
+                    double* fl, double* fr) {
+    if (int rc = recon_args(c, dir, f, fn1, fn2, fn3, lnx, lny, lnz, fl, fr)) return rc;
     hipSetDevice(c->p.device);
     const size_t nf = (size_t)fn1 * fn2 * fn3, nl = (size_t)lnx * lny * lnz;
     double *df = nullptr, *dl = nullptr, *dr = nullptr;
@@ -1327,6 +1361,28 @@ int mhd_reconstruct(mhd_ctx* c, int dir, const double* f, int fn1, int fn2, int 
     hipFree(dl);
     hipFree(dr);
     return rc;
+}
+
+// reconstruct_3d staged through this GPU's scratch: f, fl and fr take its
+// first nf, next nl and last nl doubles (each at a 256-byte boundary)
+int mhd_reconstruct_noalloc(mhd_ctx* c, int dir, const double* f, int fn1, int fn2, int fn3, int lnx, int lny,
+                            int lnz, double* fl, double* fr) {
+    if (int rc = recon_args(c, dir, f, fn1, fn2, fn3, lnx, lny, lnz, fl, fr)) return rc;
+    const long nf = (long)fn1 * fn2 * fn3, nl = (long)lnx * lny * lnz;
+    const long nf32 = (nf + 31) & ~31L, nl32 = (nl + 31) & ~31L;
+    if (nf32 + 2 * nl32 > kScratchDoubles) return MHD_ERR_PARAM;
+    if (hipSetDevice(c->p.device) != hipSuccess) return MHD_ERR_HIP;
+    double* base = scratch_on(c->p.device);
+    if (!base) return MHD_ERR_HIP;
+    double *df = base, *dl = base + nf32, *dr = base + nf32 + nl32;
+    if (hipMemcpyAsync(df, f, nf * sizeof(double), hipMemcpyHostToDevice, c->s) != hipSuccess) return MHD_ERR_HIP;
+    launch_recon(c, dir, df, fn1, fn2, lnx, lny, lnz, dl, dr);
+    int rc = MHD_OK;
+    if (hipMemcpyAsync(fl, dl, nl * sizeof(double), hipMemcpyDeviceToHost, c->s) != hipSuccess ||
+        hipMemcpyAsync(fr, dr, nl * sizeof(double), hipMemcpyDeviceToHost, c->s) != hipSuccess)
+        rc = MHD_ERR_HIP;
+    const int r2 = sync_status(c);
+    return rc == MHD_OK ? r2 : rc;
 }
 
 int mhd_v_face(mhd_ctx* c, long n, const double* lam, const double* v, double* v0p, double* v0n, double* v1p,

@@ -66,6 +66,8 @@ SIGNATURES = {
     "mhd_advance": (C.c_int, [C.c_void_p, C.c_int]),
     "mhd_fetch": (C.c_int, [C.c_void_p, C.c_int, _D]),
     "mhd_store": (C.c_int, [C.c_void_p, C.c_int, _D]),
+    "mhd_reconstruct_noalloc": (C.c_int, [C.c_void_p, C.c_int, _D, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                          C.c_int, _D, _D]),
     "mhd_v_face": (C.c_int, [C.c_void_p, C.c_long, _D, _D, _D, _D, _D, _D]),
     "mhd_reconstruct": (C.c_int, [C.c_void_p, C.c_int, _D, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                   _D, _D]),
@@ -369,13 +371,15 @@ class Ensemble:
         self.close()
 
 
-def reconstruct(solver, f, lshape, direction):
-    """reconstruct_3d of a host array through the library."""
+def reconstruct(solver, f, lshape, direction, noalloc=False):
+    """reconstruct_3d of a host array through the library (noalloc: the
+    entry point that stages through the GPU's fixed scratch array)"""
     f = np.asfortranarray(f, dtype=np.float64)
     fl = np.zeros(lshape, order="F")
     fr = np.zeros(lshape, order="F")
-    _ck(lib().mhd_reconstruct(solver.h, direction, _p(f), f.shape[0], f.shape[1], f.shape[2], lshape[0], lshape[1],
-                              lshape[2], _p(fl), _p(fr)), "mhd_reconstruct")
+    fn = lib().mhd_reconstruct_noalloc if noalloc else lib().mhd_reconstruct
+    _ck(fn(solver.h, direction, _p(f), f.shape[0], f.shape[1], f.shape[2], lshape[0], lshape[1], lshape[2], _p(fl),
+           _p(fr)), "mhd_reconstruct_noalloc" if noalloc else "mhd_reconstruct")
     return fl, fr
 
 
