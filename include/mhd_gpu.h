@@ -278,6 +278,17 @@ int mhd_profile_classes(mhd_ctx *ctx, unsigned mask);
  * up to the step).  The results are the same bit for bit either way. */
 int mhd_set_stage_overlap(mhd_ctx *ctx, int on);
 
+/* kernel fusion (no counterpart in the reference): on, a mixed-precision
+ * Adams-Bashforth step with plain PDM on the tensor-product or affine grid
+ * forms the fp32 face fluxes and applies the binary64 corrector in one
+ * kernel, so the face products never reach HBM; off (the default: on
+ * MI355X the fused kernel is slightly slower, its register footprint
+ * leaving two wavefronts per SIMD), the flux kernels store the face
+ * products and the update kernel reads them back.  The two agree to the
+ * rounding of the fp32 flux (the fused kernel's floating-point contraction
+ * may round some face values differently in the last place). */
+int mhd_set_fusion(mhd_ctx *ctx, int on);
+
 /* ---- learned face reconstruction (no counterpart in the reference) ------
  * A net y = lin(x) + gate(x) * MLP(x), MLP 40 -> 32 -> 32 -> 16 (activation
  * z*sigmoid(1.702 z)); per variable lin is the fixed 4th-order interpolation

@@ -5,6 +5,7 @@ Translation units and their floating-point contracts:
   mhd_ops_fp64.hip   binary64 flux / EMF / reconstruct  -ffp-contract=off
   mhd_ops_fp32.hip   fp32 flux / EMF (mixed precision)  -ffp-contract=fast
   mhd_surrogate.hip  learned reconstruction (bf16 MFMA) -ffp-contract=fast
+  mhd_fused.hip      fused fp32 flux + binary64 update  -ffp-contract=fast (update: contraction off)
 Packed-fp32 instructions are disabled for all device code.
 """
 import os
@@ -30,6 +31,7 @@ UNITS = {
     "mhd_ops_fp64.hip": ["-ffp-contract=off"],
     "mhd_ops_fp32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "mhd_surrogate.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "mhd_fused.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
 }
 
 

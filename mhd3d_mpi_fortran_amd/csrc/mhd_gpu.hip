@@ -14,6 +14,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "mhd_fused.hpp"
 #include "mhd_launch.hpp"
 #include "mhd_step_kernels.hpp"
 
@@ -854,6 +855,69 @@ static void stage_overlapped(mhd_ctx* c, E emf, F flux, double* const* out) {
     if (!after_ct) hipStreamWaitEvent(c->s, c->ev_join, 0);
 }
 
+// the fused face flux + update (mhd_fused.hip) takes the mixed-precision
+// Adams-Bashforth stage with plain PDM on the tensor and affine grids
+static bool can_fuse(const mhd_ctx* c) {
+    // (its CFL partials, one per workgroup, fit the partials buffer: a
+    // workgroup covers at most 64 x 4 x 32 cells, the buffer holds one slot
+    // per 256 ghost-inclusive cells)
+    return c->fused && mixed(c) && c->p.stepop == 1 && c->p.reconop == 1 && !c->p.surrogate && c->met.mode != 2 &&
+           (long)fused_blocks(c->d) <= c->ndtpart;
+}
+
+// Adams-Bashforth stage with the fused flux + update: the edge EMF and CT on
+// the second stream, the fused kernel on the main one (after CT when it
+// takes the CFL partials of CT's new faces)
+static void stage_fused(mhd_ctx* c, double* const* out) {
+    const bool dt = fuse_dt(c);
+    FusedArgs a{};
+    a.d = c->d;
+    a.c = make_coef<float>(c);
+    const Src<float> sh = sh_src(c);
+    a.q = Prim<float>{sh.rho, sh.vx, sh.vy, sh.vz, sh.p, sh.bx, sh.by, sh.bz};
+    for (int r = 0; r < 3; ++r)
+        for (int q = 0; q < 3; ++q) a.n[r][q] = float(c->met.n[r][q]);
+    a.rho = c->rho;
+    a.vx = c->vx;
+    a.vy = c->vy;
+    a.vz = c->vz;
+    a.p = c->pr;
+    a.orho = out[0];
+    a.ovx = out[1];
+    a.ovy = out[2];
+    a.ovz = out[3];
+    a.op = out[4];
+    a.dx = sdx(c);
+    a.dy = sdy(c);
+    a.dz = sdz(c);
+    a.tau_p = c->scal + SC_TAU;
+    a.gm1 = c->p.gam - 1.;
+    a.bi = out[8];
+    a.bj = out[9];
+    a.bk = out[10];
+    for (int q = 0; q < 9; ++q) a.M[q] = c->met.M[q];
+    a.met_on = c->met.on ? 1 : 0;
+    a.gam = c->p.gam;
+    a.cfl = c->p.cfl;
+    a.part = c->dtpart;
+    const bool ok = c->overlap && hipEventRecord(c->ev_fork, c->s) == hipSuccess &&
+                    hipStreamWaitEvent(c->s2, c->ev_fork, 0) == hipSuccess;
+    if (!ok) {
+        launch_emf_fp32(c, sh);
+        do_ct_t<float, 0, 0>(c, out);
+        launch_flux_update(c, a, dt);
+        return;
+    }
+    c->cur = c->s2;
+    launch_emf_fp32(c, sh);
+    do_ct_t<float, 0, 0>(c, out);
+    hipEventRecord(c->ev_join, c->s2);
+    c->cur = nullptr;
+    if (dt) hipStreamWaitEvent(c->s, c->ev_join, 0);
+    launch_flux_update(c, a, dt);
+    if (!dt) hipStreamWaitEvent(c->s, c->ev_join, 0);
+}
+
 template <int NST, int ST>
 static void do_stage(mhd_ctx* c, Src<double> s64, double* const* out) {
     if (mixed(c)) {
@@ -1045,7 +1109,9 @@ static void predict_ab(mhd_ctx* c) {
 static void step_ab(mhd_ctx* c) {
     do_dt(c, true);
     predict_ab(c);
-    do_stage<0, 0>(c, h_src(c), c->xn);
+    const bool fused = can_fuse(c);
+    if (fused) stage_fused(c, c->xn);
+    else do_stage<0, 0>(c, h_src(c), c->xn);
     // rotate: previous <- current <- next <- previous
     for (int a = 0; a < 11; ++a) {
         double* cur = *state_slot(c, a);
@@ -1056,7 +1122,7 @@ static void step_ab(mhd_ctx* c) {
     do_fill(c);
     if (fuse_dt(c)) {  // the update left the CFL partials of the new state
         c->dt_fresh = true;
-        c->ndt_valid = blocks(c->ninterior).x;
+        c->ndt_valid = fused ? fused_blocks(c->d) : blocks(c->ninterior).x;
         c->cellb_stale = true;
     } else {
         end_of_stage(c, true);
@@ -1470,6 +1536,12 @@ int mhd_profile(mhd_ctx* c, int enable) {
 int mhd_set_stage_overlap(mhd_ctx* c, int on) {
     if (!c) return MHD_ERR_PARAM;
     c->overlap = on != 0;
+    return MHD_OK;
+}
+
+int mhd_set_fusion(mhd_ctx* c, int on) {
+    if (!c) return MHD_ERR_PARAM;
+    c->fused = on != 0;
     return MHD_OK;
 }
 

@@ -89,6 +89,42 @@ __device__ __forceinline__ bool tile_face(unsigned tile, int ii, int jj, int kk,
     return x < (unsigned)ii && s < nn && o < no;
 }
 
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+// minimum over the workgroup (every thread calls it; two barriers)
+__device__ __forceinline__ double block_min(double v) {
+    __shared__ double red[kBlock / 64];
+    v = wave_min(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = red[q] < m ? red[q] : m;
+        red[0] = m;
+    }
+    __syncthreads();
+    return red[0];
+}
+
+// the CFL step of one cell (get_dt, mhd3d3.f90:1608-1628)
+__device__ __forceinline__ double cfl_dt(double gam, double cfl, double u, double v, double w, double x, double y,
+                                         double z, double r, double pp, double dxi, double dyj, double dzk) {
+    const double vv = sqrt(u * u + v * v + w * w);
+    const double btot = sqrt(x * x + y * y + z * z);
+    const double valf = btot / sqrt(r);
+    const double vs = sqrt(gam * pp / r);
+    const double vcfl = vv + sqrt(valf * valf + vs * vs);
+    return cfl / (vcfl / dxi + vcfl / dyj + vcfl / dzk);
+}
+
 template <typename T>
 struct Prim {  // cell-centred state read by the flux and EMF kernels
     const T *rho, *vx, *vy, *vz, *p, *bx, *by, *bz;
@@ -101,9 +137,8 @@ struct Prim {  // cell-centred state read by the flux and EMF kernels
 
 // Gaussian-split flux of one face from its left/right states
 // l, r = (rho, vx, vy, vz, p, bx, by, bz), written at index t of F
-template <typename R, int D, typename TF>
-__device__ __forceinline__ void face_flux(const Coef<R>& c, const R* l, const R* r, TF* __restrict__ F, long t,
-                                          long nface) {
+template <typename R, int D>
+__device__ __forceinline__ void face_flux_vals(const Coef<R>& c, const R* l, const R* r, R* out) {
     const R rl = l[0], ul = l[1], vl = l[2], wl = l[3], pl = l[4], bxl = l[5], byl = l[6], bzl = l[7];
     const R rr = r[0], ur = r[1], vr = r[2], wr = r[3], pr = r[4], bxr = r[5], byr = r[6], bzr = r[7];
     R v0, v1;
@@ -145,14 +180,23 @@ __device__ __forceinline__ void face_flux(const Coef<R>& c, const R* l, const R*
     R ty = (D == 2 ? (R(.5) * bsq - byr * byr) : (-byr * bnr)) * v0;
     R tz = (D == 3 ? (R(.5) * bsq - bzr * bzr) : (-bzr * bnr)) * v0;
 
-    F[t] = TF(fr0 + gr0);
-    F[nface + t] = TF(fmx + gmx);
-    F[2 * nface + t] = TF(fmy + gmy);
-    F[3 * nface + t] = TF(fmz + gmz);
-    F[4 * nface + t] = TF(fe + ge);
-    F[5 * nface + t] = TF(sx + tx);
-    F[6 * nface + t] = TF(sy + ty);
-    F[7 * nface + t] = TF(sz + tz);
+    out[0] = fr0 + gr0;
+    out[1] = fmx + gmx;
+    out[2] = fmy + gmy;
+    out[3] = fmz + gmz;
+    out[4] = fe + ge;
+    out[5] = sx + tx;
+    out[6] = sy + ty;
+    out[7] = sz + tz;
+}
+
+template <typename R, int D, typename TF>
+__device__ __forceinline__ void face_flux(const Coef<R>& c, const R* l, const R* r, TF* __restrict__ F, long t,
+                                          long nface) {
+    R o[8];
+    face_flux_vals<R, D>(c, l, r, o);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) F[q * nface + t] = TF(o[q]);
 }
 
 // ------------------------------------------------------------------------
@@ -184,11 +228,9 @@ struct Normals {
     R n[3][3];  // n[d] = unit normal of the faces normal to computational axis d
 };
 
-template <typename R, typename TF>
-__device__ __forceinline__ void face_flux_met(const Coef<R>& c, const R* l, const R* r, const R* n,
-                                              TF* __restrict__ F, long t, long nface) {
+template <typename R>
+__device__ __forceinline__ void face_flux_met_vals(const Coef<R>& c, const R* l, const R* r, const R* n, R* fx) {
     R v0, v1;
-    R fx[8];
     // left state, positive half-moments
     {
         const R rl = l[0], ul = l[1], vl = l[2], wl = l[3], pl = l[4], bxl = l[5], byl = l[6], bzl = l[7];
@@ -229,6 +271,13 @@ __device__ __forceinline__ void face_flux_met(const Coef<R>& c, const R* l, cons
         fx[6] += (R(.5) * bsq * n[1] - byr * bn) * v0;
         fx[7] += (R(.5) * bsq * n[2] - bzr * bn) * v0;
     }
+}
+
+template <typename R, typename TF>
+__device__ __forceinline__ void face_flux_met(const Coef<R>& c, const R* l, const R* r, const R* n,
+                                              TF* __restrict__ F, long t, long nface) {
+    R fx[8];
+    face_flux_met_vals<R>(c, l, r, n, fx);
 #pragma unroll
     for (int q = 0; q < 8; ++q) F[q * nface + t] = TF(fx[q]);
 }

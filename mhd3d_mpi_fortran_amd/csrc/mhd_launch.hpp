@@ -56,6 +56,7 @@ struct mhd_ctx {
     hipStream_t s2 = nullptr, cur = nullptr;
     hipEvent_t ev_fork = nullptr, ev_emf = nullptr, ev_join = nullptr;
     bool overlap = true;   // false: one stream, kernels in the reference's order
+    bool fused = false;    // mixed Adams-Bashforth PDM steps: fused face flux + update (mhd_set_fusion)
     char* arena = nullptr;
     size_t arena_bytes = 0;
     bool have_geometry = false;

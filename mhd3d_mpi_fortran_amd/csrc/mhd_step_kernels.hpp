@@ -30,39 +30,6 @@ struct FluxSet {
     long nfx, nfy, nfz;
 };
 
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const double w = __shfl_xor(v, o, 64);
-        v = w < v ? w : v;
-    }
-    return v;
-}
-
-__device__ __forceinline__ double block_min(double v) {
-    __shared__ double red[kBlock / 64];
-    v = wave_min(v);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) red[wid] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = red[0];
-        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = red[q] < m ? red[q] : m;
-        red[0] = m;
-    }
-    __syncthreads();
-    return red[0];
-}
-
-__device__ __forceinline__ double cfl_dt(double gam, double cfl, double u, double v, double w, double x, double y,
-                                         double z, double r, double pp, double dxi, double dyj, double dzk) {
-    const double vv = sqrt(u * u + v * v + w * w);
-    const double btot = sqrt(x * x + y * y + z * z);
-    const double valf = btot / sqrt(r);
-    const double vs = sqrt(gam * pp / r);
-    const double vcfl = vv + sqrt(valf * valf + vs * vs);
-    return cfl / (vcfl / dxi + vcfl / dyj + vcfl / dzk);
-}
 
 // CFL partials of the updated state, computed by the update itself (get_dt
 // on the new state, mhd3d3.f90:1608-1639): cell B from the stage's new

@@ -78,6 +78,7 @@ SIGNATURES = {
     "mhd_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]),
     "mhd_profile_classes": (C.c_int, [C.c_void_p, C.c_uint]),
     "mhd_set_stage_overlap": (C.c_int, [C.c_void_p, C.c_int]),
+    "mhd_set_fusion": (C.c_int, [C.c_void_p, C.c_int]),
     "mhd_divb_max": (C.c_int, [C.c_void_p, _D]),
     "mhd_ensemble_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int, _D]),
     "mhd_ensemble_step": (C.c_int, [C.c_void_p, C.c_int]),
@@ -322,6 +323,10 @@ class Solver:
     def set_overlap(self, on=True):
         """EMF and CT on a second stream beside the flux and update (default)"""
         _ck(lib().mhd_set_stage_overlap(self.h, 1 if on else 0), "mhd_set_stage_overlap")
+
+    def set_fusion(self, on=True):
+        """fused face flux + update of mixed Adams-Bashforth PDM steps (off by default)"""
+        _ck(lib().mhd_set_fusion(self.h, 1 if on else 0), "mhd_set_fusion")
 
     def profile_classes(self, classes):
         """time only the given kernel classes (names or MHD_K_* codes)"""
