@@ -132,7 +132,7 @@ inline hipEvent_t ev_get(mhd_ctx* c) {
         return e;
     }
     hipEvent_t e = nullptr;
-    hipEventCreate(&e);
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
     return e;
 }
 
@@ -142,17 +142,23 @@ inline void launch(mhd_ctx* c, int kclass, dim3 g, dim3 b, void (*k)(KArgs...), 
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const bool timed = c->prof && ((c->prof_mask >> kclass) & 1u);
     hipStream_t st = c->cur ? c->cur : c->s;
+    auto note = [c](hipError_t err) {
+        if (err != hipSuccess && c->launch_error == hipSuccess) c->launch_error = err;
+    };
     if (timed) {
         e0 = ev_get(c);
         e1 = ev_get(c);
-        hipEventRecord(e0, st);
+        if (!e0 || !e1) note(hipErrorOutOfMemory);
+        else note(hipEventRecord(e0, st));
     }
     hipLaunchKernelGGL(k, g, b, 0, st, static_cast<KArgs>(args)...);
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess && c->launch_error == hipSuccess) c->launch_error = err;
-    if (timed) {
-        hipEventRecord(e1, st);
+    note(hipGetLastError());
+    if (timed && e0 && e1) {
+        note(hipEventRecord(e1, st));
         c->evs.push_back({kclass, e0, e1});
+    } else {
+        if (e0) c->evpool.push_back(e0);
+        if (e1) c->evpool.push_back(e1);
     }
 }
 

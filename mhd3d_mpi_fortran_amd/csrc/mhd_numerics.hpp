@@ -52,6 +52,20 @@ template <typename R>
 __device__ __forceinline__ R dmax(R a, R b) { return a > b ? a : b; }
 template <typename R>
 __device__ __forceinline__ R dmin(R a, R b) { return a < b ? a : b; }
+// fp32: the hardware max/min, one instruction instead of a compare and a
+// select; the same value as the comparisons for every pair of numbers but
+// +0/-0 (either may come back), and a NaN operand gives the other operand
+template <>
+__device__ __forceinline__ float dmax<float>(float a, float b) { return __builtin_fmaxf(a, b); }
+template <>
+__device__ __forceinline__ float dmin<float>(float a, float b) { return __builtin_fminf(a, b); }
+
+// (1+d)/2 a + (1-d)/2 b for d = SIGN(1.,x) = +-1 (mhd3d3.f90:1823-1824,
+// 2106-2109); in fp32 the select it equals for finite a and b
+template <typename R>
+__device__ __forceinline__ R pick(R d, R a, R b) { return (R(1) + d) / R(2) * a + (R(1) - d) / R(2) * b; }
+template <>
+__device__ __forceinline__ float pick<float>(float d, float a, float b) { return d > 0.f ? a : b; }
 
 // interp_pdm 8-point face value, s[0..7] = cells -4..+3 (mhd3d3.f90:871-874)
 template <typename R>
@@ -227,6 +241,26 @@ __device__ __forceinline__ T load_at(const T* __restrict__ f, unsigned byte_off)
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(f) + byte_off);
 }
 
+// raw buffer view of one array below 2^32 bytes: a load takes a per-lane byte
+// offset (a VGPR) and a uniform byte offset (an SGPR), so the taps of a
+// stencil at a uniform stride from the lane's base cost no address
+// arithmetic.  A load past the array's end returns 0 instead of reading
+// beyond it.
+template <typename T>
+struct Buf {
+    // descriptor word 3 of gfx9 buffers: 32-bit data format, no swizzle
+    static constexpr int kWord3 = 0x00020000;
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ Buf(const T* p, unsigned bytes)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(p), (short)0, (int)bytes, kWord3)) {}
+    __device__ __forceinline__ T at(unsigned voff, unsigned soff) const {
+        if constexpr (sizeof(T) == 4)
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+        else
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    }
+};
+
 // recon_line with precomputed tap offsets
 template <typename R, int M, typename T>
 __device__ __forceinline__ void recon_taps(const T* __restrict__ f, const Taps<R, M, T>& tp, const Coef<R>& c,
@@ -257,6 +291,19 @@ __device__ __forceinline__ void recon_off(const T* __restrict__ f, unsigned ob, 
         for (int m = 0; m < S::n; ++m)
             s[m] = R(load_at(reinterpret_cast<const T*>(b + (ptrdiff_t)(S::lo + m) * (ptrdiff_t)ost), ob));
     }
+    S::run(s, c, fl, fr);
+}
+
+// recon_line through a buffer view: the taps at ob + (lo+m)*ost share one
+// offset register, the uniform part going into each load's scalar offset
+template <typename R, int M, typename T>
+__device__ __forceinline__ void recon_buf(const Buf<T>& f, unsigned ob, unsigned ost, const Coef<R>& c, R& fl,
+                                          R& fr) {
+    using S = Stencil<R, M>;
+    R s[S::n];
+    const unsigned o0 = ob + (unsigned)S::lo * ost;  // modulo 2^32: the first tap's offset
+#pragma unroll
+    for (int m = 0; m < S::n; ++m) s[m] = R(f.at(o0, (unsigned)m * ost));
     S::run(s, c, fl, fr);
 }
 

@@ -56,7 +56,8 @@ void emf_launch_kt(mhd_ctx* c, EmfIn<TI> in) {
     const long no = K == 3 ? d.nz0 : (K == 1 ? d.ny0 + 1 : d.ny0);
     const long nm = K == 3 ? d.ny0 + 1 : d.nz0 + 1;
     const long n = ii * no * ((nm + T - 1) / T);
-    launch(c, MHD_K_EMF, blocks(n), dim3(kBlock), k_emf_march<R, M, K, T, TI, TF>, d, make_coef<R>(c), in,
+    const dim3 grid = K == 1 ? dim3(emf_tiles<T>(d.nx0, d.ny0, d.nz0)) : blocks(n);
+    launch(c, MHD_K_EMF, grid, dim3(kBlock), k_emf_march<R, M, K, T, TI, TF>, d, make_coef<R>(c), in,
            R(c->met.ef[K - 1]), c->met.mode == 2 ? c->gm.ef[K - 1] : (const double*)nullptr, (TF*)c->emf[K - 1], n);
 }
 
@@ -67,7 +68,10 @@ void emf_launch_k(mhd_ctx* c, EmfIn<TI> in) {
     const long ii = d.nx0 + (K != 1);
     const long no = K == 3 ? d.nz0 : (K == 1 ? d.ny0 + 1 : d.ny0);
     const long nm = K == 3 ? d.ny0 + 1 : d.nz0 + 1;
-    if (ii * no * ((nm + T - 1) / T) >= c->emf_fill) emf_launch_kt<R, M, K, T, TI, TF>(c, in);
+    // a run holds exactly T edges (the last one of a line overlaps its
+    // predecessor), so long runs need a march axis of at least T edges; it
+    // holds at least 2 = T_SHORT
+    if (nm >= T && ii * no * ((nm + T - 1) / T) >= c->emf_fill) emf_launch_kt<R, M, K, T, TI, TF>(c, in);
     else emf_launch_kt<R, M, K, EmfSteps<M, K>::T_SHORT, TI, TF>(c, in);
 }
 

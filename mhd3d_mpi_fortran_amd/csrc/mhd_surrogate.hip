@@ -898,13 +898,13 @@ static SurrogateState* sstate(mhd_ctx* c) {
             delete s;
             return nullptr;
         }
-        hipMemsetAsync(s->g, 0, pb, c->s);
-        hipMemsetAsync(s->m, 0, pb, c->s);
-        hipMemsetAsync(s->v, 0, pb, c->s);
-        hipMemsetAsync(s->p, 0, pb, c->s);
-        hipMemsetAsync(s->count, 0, 64, c->s);
-        hipMemsetAsync(s->stats, 0, 64, c->s);
         c->sur = s;
+        if (hipMemsetAsync(s->g, 0, pb, c->s) != hipSuccess || hipMemsetAsync(s->m, 0, pb, c->s) != hipSuccess ||
+            hipMemsetAsync(s->v, 0, pb, c->s) != hipSuccess || hipMemsetAsync(s->p, 0, pb, c->s) != hipSuccess ||
+            hipMemsetAsync(s->count, 0, 64, c->s) != hipSuccess || hipMemsetAsync(s->stats, 0, 64, c->s) != hipSuccess) {
+            surrogate_free(c);
+            return nullptr;
+        }
     }
     return c->sur;
 }

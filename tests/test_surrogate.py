@@ -222,6 +222,38 @@ def test_gpu_learned_flux_matches_oracle(kind):
         assert fb > 0
 
 
+@pytest.mark.gpu
+def test_gpu_linear_part_matches_oracle_across_ring_runs():
+    """zero weights and no threshold: every face takes the clamped linear part
+    (the MLP adds gate * 0), so the learned flux must equal the oracle's
+    learned flux to fp32 rounding on every face.  The y and z sweeps read
+    their windows from an LDS ring of rows that a workgroup marches through
+    16 planes; 37 y faces and 35 z faces give each sweep three runs (the last
+    one partial), so a ring slot or row off by one would move a face's
+    states by a cell's difference, O(1e-2) of the flux here."""
+    g = ics.Grid(20, 36, 34, ng=4)
+    st = ics.initial_state(g, "orszag_tang", 34)
+    w = np.zeros(sg.NPARAM, np.float32)
+    s = gpu.Solver(gpu.make_params(20, 36, 34, ng=4, reconop=1, stepop=1, bc="even", precision="mixed",
+                                   surrogate=1), g)
+    sg.set_weights(s, w)
+    sg.configure(s, 1e30, count=True)
+    s.upload(st)
+    s.fill()
+    f = s.flux_stress()
+    faces, fb = sg.stats(s)
+    s.close()
+    _, ref = _oracle_pair(g, st, w, 1e30)
+    assert faces > 0 and fb == 0, (faces, fb)
+    worst = 0.
+    for q in range(24):
+        scale = np.abs(ref[q]).max() + 1e-300
+        err = float(np.abs(f[q] - ref[q]).max()) / scale
+        worst = max(worst, err)
+        assert err <= 1e-5, (q, err)
+    print("linear part vs oracle: largest flux error %.2e of the array's scale" % worst)
+
+
 TRAIN_THETA = .5
 
 
