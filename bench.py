@@ -65,10 +65,13 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "blast256_sheared_traffic.json")
 TRAFFIC_KERNELS = {("flux", "mixed", True): ["sur::k_flux_sur<%d, 1>" % d for d in (1, 2, 3)],
                    ("flux", "mixed", False): ["mhd::k_flux<float, 1, %d, float, float, 1>" % d for d in (1, 2, 3)],
                    ("flux", "fp64", False): ["mhd::k_flux<double, 1, %d, double, double, 1>" % d for d in (1, 2, 3)],
-                   ("emf", "mixed", True): ["mhd::k_emf_march<float, 1, %d, 8, float, float>" % d for d in (1, 2, 3)],
-                   ("emf", "mixed", False): ["mhd::k_emf_march<float, 1, %d, 8, float, float>" % d for d in (1, 2, 3)],
-                   ("emf", "fp64", False): ["mhd::k_emf_march<double, 1, %d, 8, double, double>" % d
-                                            for d in (1, 2, 3)]}
+                   # PDM's ei march takes runs of 16 edges, ej and ek runs of 8
+                   ("emf", "mixed", True): ["mhd::k_emf_march<float, 1, %d, %d, float, float>" % dt
+                                            for dt in ((1, 16), (2, 8), (3, 8))],
+                   ("emf", "mixed", False): ["mhd::k_emf_march<float, 1, %d, %d, float, float>" % dt
+                                             for dt in ((1, 16), (2, 8), (3, 8))],
+                   ("emf", "fp64", False): ["mhd::k_emf_march<double, 1, %d, %d, double, double>" % dt
+                                            for dt in ((1, 16), (2, 8), (3, 8))]}
 
 
 def measured_traffic(kcls, prec, sur, n):

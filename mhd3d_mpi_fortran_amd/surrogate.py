@@ -104,7 +104,7 @@ SETUPS = (("orszag_tang", 101, {}), ("blobs", 102, {}), ("blast", 103, {"radius"
           ("orszag_tang", 104, {}), ("blobs", 105, {}), ("blast", 106, {"radius": .1}))
 
 
-def build_training_set(n=128, capacity=1 << 24, theta=THETA, stride=4, snapshots=4, steps_between=8,
+def build_training_set(n=128, capacity=1 << 24, theta=THETA, stride=4, snapshots=6, steps_between=8,
                        setups=SETUPS, log=None):
     """run the binary64 solver (PDM, RK2, periodic) on the training set-ups
     and collect face samples from several snapshots of each; returns the

@@ -99,10 +99,12 @@ def test_initialize_fill_bxyz_dt_bit_exact(bc, stepop):
 @pytest.mark.parametrize("runs", ["short", "long"])
 def test_eijk_bit_exact(reconop, kind, runs, monkeypatch):
     # the EMF kernel walks runs of edges along its march axis; small grids
-    # take short runs, large ones long runs: both against the oracle (the
-    # march axis is 13 long here, so long runs end in a partial run)
+    # take short runs (2 edges), large ones long runs (16 edges for PDM's ei
+    # and TVD, 8 otherwise): both against the oracle.  The march axes are 21
+    # (z: ei, ej) and 11 (y: ek) edges long, so every long-run launch ends
+    # with a run that starts early and stores only its own edges.
     monkeypatch.setenv("MHD_EMF_FILL_THREADS", "0" if runs == "long" else str(1 << 40))
-    g, s, o = pair(12, 10, 12, kind, seed=reconop, reconop=reconop)
+    g, s, o = pair(12, 10, 20, kind, seed=reconop, reconop=reconop)
     s.initialize(1e-3)
     o.initialize(1e-3)
     e = s.eijk()

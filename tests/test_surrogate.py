@@ -315,24 +315,34 @@ def test_trained_net_gains_over_baselines(trained):
 
 @pytest.mark.gpu
 def test_solver_with_learned_reconstruction(trained):
-    """48^3 Orszag-Tang, AB2 (CFL dt): the mixed solver with the trained net against
-    the binary64 solver, next to the same solver with the linear part only
-    (zero MLP) -- what the trained correction gains inside the solver."""
+    """48^3 Orszag-Tang, AB2 (CFL dt): the mixed solver with the trained net
+    against the binary64 solver, next to the same solver with the linear part
+    only (zero MLP) and the mixed PDM solver.
+
+    What this measures, stated plainly: the trained correction does not beat
+    its own linear part inside the solver.  At theta = 0.25 the seeded density
+    noise of this set-up sends ~99 % of the faces to the PDM fallback, the net
+    acts on the rest, and its density L1 sits at or above the linear part's
+    (1.37x trained here on 2^20 samples; 1.00x after 8 epochs on 2^24); both
+    are two orders of magnitude above fp32 PDM.  The bounds below
+    are regression guards on that state, not a claimed gain: the net within
+    1.5x of the linear part and below 1e-4, PDM below both."""
     n = 48
     g = ics.Grid(n, n, n, ng=4)
     st = ics.initial_state(g, "orszag_tang", 903)
     ref = gpu.Solver(gpu.make_params(n, n, n, reconop=1, stepop=1, bc="periodic"), g)
     runs = {}
     zero = np.zeros(sg.NPARAM, np.float32)
-    for name, w in (("net", trained["w"]), ("linear", zero)):
+    for name, w in (("net", trained["w"]), ("linear", zero), ("pdm", None)):
         sur = gpu.Solver(gpu.make_params(n, n, n, reconop=1, stepop=1, bc="periodic", precision="mixed",
-                                         surrogate=1), g)
-        sg.set_weights(sur, w)
-        sg.configure(sur, sg.THETA, count=True)
+                                         surrogate=0 if w is None else 1), g)
+        if w is not None:
+            sg.set_weights(sur, w)
+            sg.configure(sur, sg.THETA, count=True)
         sur.upload(st)
         tau = sur.initialize(1e-4)
         sur.step(20, tau)
-        runs[name] = (sur.download(), sg.stats(sur), sur.check_state())
+        runs[name] = (sur.download(), sg.stats(sur) if w is not None else (1, 0), sur.check_state())
         sur.close()
     ref.upload(st)
     tau = ref.initialize(1e-4)
@@ -346,11 +356,9 @@ def test_solver_with_learned_reconstruction(trained):
                                                                                100. * fb / max(faces, 1), faces))
         assert faces > 0 and fb < faces
         assert flags == 0
-    # the fallback threshold bounds the error: at this threshold the linear
-    # part is already close to PDM on the faces the net sees, and the trained
-    # correction must keep the solver at that level
     assert l1["net"] <= 1e-4
-    assert l1["net"] <= 2. * l1["linear"]
+    assert l1["net"] <= 1.5 * l1["linear"]
+    assert l1["pdm"] < min(l1["net"], l1["linear"])
 
 
 @pytest.mark.gpu
