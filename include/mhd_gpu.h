@@ -167,6 +167,20 @@ enum { MHD_STATE_CURRENT = 0, MHD_STATE_PREDICTED = 1 };
 int mhd_flux_stress(mhd_ctx *ctx, int which_state);   /* get_flux_stress, 2385-2560 */
 int mhd_eijk_lfm(mhd_ctx *ctx, int which_state);      /* get_eijk_LFM, 1754-1856 */
 
+/* the reference's dead alternatives (their calls, mhd3d3.f90:3354 and 3621,
+ * are commented out; the second has none):
+ * get_eijk_Yee (1654-1752, with get_v_center 2562-2592): the EMF from the
+ * cell-centred electric field split by the Gaussian half-moments and
+ * reconstructed face then edge (reconstruct_3d directions 4-6), into the
+ * context's ei/ej/ek like mhd_eijk_lfm (mhd_fetch, mhd_advance read them).
+ * Binary64 contexts on the tensor-product grid (MHD_ERR_UNSUPPORTED
+ * otherwise); the ghost layers must be filled, as for every routine here.
+ * get_flux_stress_old (2131-2383): get_flux_stress's arithmetic, operation
+ * for operation, on the Adams-Bashforth extrapolation rhoh, vxh, ... =
+ * mhd_flux_stress(ctx, MHD_STATE_PREDICTED). */
+int mhd_eijk_yee(mhd_ctx *ctx, int which_state);
+int mhd_flux_stress_old(mhd_ctx *ctx);
+
 /* the module variables tau and tau0 (module pars, mhd3d3.f90:33-248): the
  * step the update routines use and the previous one (both > 0); e.g. the
  * reference's `tau=tau0` of step_rk2 (3662) */

@@ -6,6 +6,7 @@
 // step_ab / step_rk2 / step_rk3 (mhd3d3.f90:3602-3792) kernel for kernel.
 #include "../../include/mhd_gpu.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1010,6 +1011,86 @@ int mhd_eijk_lfm(mhd_ctx* c, int which_state) {
     }
     return sync_status(c);
 }
+
+// get_eijk_Yee (mhd3d3.f90:1654-1752).  The reference builds the twelve
+// split fields over the whole block, then for each edge direction reconstructs
+// four of them along one transverse axis (left states of pp and pn, right
+// states of nn and np, untruncated directions 4-6), those four along the other
+// transverse axis (left of pp and np, right of nn and pn) and sums the
+// quadrants at the owned edges:
+//   ei: y (dir 5) on (nx, ny0+1, nz), then z (dir 6) on (nx, ny0+1, nz0+1)
+//   ej: z (dir 6) on (nx, ny, nz0+1), then x (dir 4) on (nx0+1, ny, nz0+1)
+//   ek: x (dir 4) on (nx0+1, ny, nz), then y (dir 5) on (nx0+1, ny0+1, nz)
+// Here the split is one kernel, each reconstruction is k_recon (the
+// reconstruct_3d pinned bit for bit) and the sums a last kernel; the working
+// arrays are allocated for the call.
+static int eijk_yee(mhd_ctx* c, Prim<double> q) {
+    const Dims& d = c->d;
+    const int nx = d.nx, ny = d.ny, nz = d.nz, nx0 = d.nx0, ny0 = d.ny0, nz0 = d.nz0;
+    const long ncell = (long)nx * ny * nz;
+    struct Pass {
+        int dir1, l1[3], dir2, l2[3], ax;
+    };
+    const Pass ps[3] = {{5, {nx, ny0 + 1, nz}, 6, {nx, ny0 + 1, nz0 + 1}, 0},
+                        {6, {nx, ny, nz0 + 1}, 4, {nx0 + 1, ny, nz0 + 1}, 1},
+                        {4, {nx0 + 1, ny, nz}, 5, {nx0 + 1, ny0 + 1, nz}, 2}};
+    long nq = 0, ne = 0;
+    for (const Pass& p : ps) {
+        nq = std::max(nq, (long)p.l1[0] * p.l1[1] * p.l1[2]);
+        ne = std::max(ne, (long)p.l2[0] * p.l2[1] * p.l2[2]);
+    }
+    // 12 split fields, 4 face states, 4 edge states, one discarded side
+    const long total = 12 * ncell + 4 * nq + 4 * ne + std::max(nq, ne);
+    double* buf = nullptr;
+    if (hipMalloc(&buf, (size_t)total * sizeof(double)) != hipSuccess) return MHD_ERR_HIP;
+    double* split[12];
+    for (int a = 0; a < 12; ++a) split[a] = buf + a * ncell;
+    double* fq[4];
+    double* fe[4];
+    for (int a = 0; a < 4; ++a) {
+        fq[a] = buf + 12 * ncell + a * nq;
+        fe[a] = buf + 12 * ncell + 4 * nq + a * ne;
+    }
+    double* waste = buf + 12 * ncell + 4 * nq + 4 * ne;
+    launch_yee_split(c, ncell, q, split);
+    const int fn[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; ++a) {
+        const Pass& p = ps[a];
+        double* const* e = split + 4 * a;
+        // face states: left of pp and pn, right of nn and np
+        launch_recon(c, p.dir1, e[YEE_PP], fn[0], fn[1], p.l1[0], p.l1[1], p.l1[2], fq[YEE_PP], waste);
+        launch_recon(c, p.dir1, e[YEE_PN], fn[0], fn[1], p.l1[0], p.l1[1], p.l1[2], fq[YEE_PN], waste);
+        launch_recon(c, p.dir1, e[YEE_NN], fn[0], fn[1], p.l1[0], p.l1[1], p.l1[2], waste, fq[YEE_NN]);
+        launch_recon(c, p.dir1, e[YEE_NP], fn[0], fn[1], p.l1[0], p.l1[1], p.l1[2], waste, fq[YEE_NP]);
+        // edge states: left of pp and np, right of nn and pn
+        launch_recon(c, p.dir2, fq[YEE_PP], p.l1[0], p.l1[1], p.l2[0], p.l2[1], p.l2[2], fe[0], waste);
+        launch_recon(c, p.dir2, fq[YEE_NP], p.l1[0], p.l1[1], p.l2[0], p.l2[1], p.l2[2], fe[1], waste);
+        launch_recon(c, p.dir2, fq[YEE_PN], p.l1[0], p.l1[1], p.l2[0], p.l2[1], p.l2[2], waste, fe[2]);
+        launch_recon(c, p.dir2, fq[YEE_NN], p.l1[0], p.l1[1], p.l2[0], p.l2[1], p.l2[2], waste, fe[3]);
+        // epp + enp + epn + enn
+        const int o[3] = {a == 0 ? nx0 : nx0 + 1, a == 1 ? ny0 : ny0 + 1, a == 2 ? nz0 : nz0 + 1};
+        const double* e4[4] = {fe[0], fe[1], fe[2], fe[3]};
+        launch_yee_sum(c, o[0], o[1], o[2], p.ax, p.l2[0], p.l2[1], e4, (double*)c->emf[a]);
+    }
+    const int rc = sync_status(c);
+    hipFree(buf);
+    return rc;
+}
+
+int mhd_eijk_yee(mhd_ctx* c, int which_state) {
+    if (!c || !c->have_geometry) return MHD_ERR_PARAM;
+    if (mixed(c) || c->met.on) return MHD_ERR_UNSUPPORTED;
+    if (int rc = check_which(c, which_state)) return rc;
+    hipSetDevice(c->p.device);
+    if (which_state == MHD_STATE_PREDICTED) {
+        const Src<double> h = h_src(c);
+        return eijk_yee(c, Prim<double>{h.rho, h.vx, h.vy, h.vz, h.p, h.bx, h.by, h.bz});
+    }
+    ensure_cellb(c);
+    return eijk_yee(c, cur_state(c));
+}
+
+int mhd_flux_stress_old(mhd_ctx* c) { return mhd_flux_stress(c, MHD_STATE_PREDICTED); }
 
 // vf and fp32 copies are kept only where the next step reads them: RK reads
 // the current state (vf in binary64, the copies in mixed precision), AB

@@ -30,6 +30,12 @@ struct Metric {
     double ef[3] = {1, 1, 1};   // |C e_a||C e_b| / det A: EMF factor of edges along d
     double M[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};  // A diag(g): face-average B -> cell B
 };
+
+// get_eijk_Yee's twelve split fields e<a><s1><s2> of edge direction a (x, y,
+// z), s1 the side of the first transverse velocity's moments, s2 the
+// second's: e_a = B_b v_b0(s1) v_c1(s2) - B_c v_b1(s1) v_c0(s2), (a, b, c)
+// cyclic; index 4*a + YEE_<s1><s2>
+enum { YEE_PP = 0, YEE_NP = 1, YEE_PN = 2, YEE_NN = 3 };
 }  // namespace mhd
 
 // device scalars: scal[SC_TAU] = tau, scal[SC_TAU0] = tau0, scal[SC_DTCFL] = get_dt result,
@@ -198,3 +204,7 @@ void launch_vface_fp64(mhd_ctx* c, long n, const double* lam, const double* v, d
 void launch_vface_fp32(mhd_ctx* c, long n, const double* lam, const double* v, double* v0p, double* v0n, double* v1p,
                        double* v1n);
 void surrogate_free(mhd_ctx* c);
+// get_eijk_Yee's split (12 ghost-inclusive cell arrays, e[4*a + YEE_*]) and
+// quadrant sums (mhd_yee.hpp)
+void launch_yee_split(mhd_ctx* c, long n, mhd::Prim<double> q, double* const* e);
+void launch_yee_sum(mhd_ctx* c, int o1, int o2, int o3, int ax, int n1, int n2, const double* const* e4, double* out);

@@ -98,6 +98,16 @@ module mhd_gpu
          type(c_ptr), value :: ctx
          integer(c_int), value :: which_state
       end function
+      ! the reference's dead alternatives get_eijk_Yee and get_flux_stress_old
+      integer(c_int) function mhd_eijk_yee(ctx, which_state) bind(C, name='mhd_eijk_yee')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: which_state
+      end function
+      integer(c_int) function mhd_flux_stress_old(ctx) bind(C, name='mhd_flux_stress_old')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
       ! the module variables tau, tau0
       integer(c_int) function mhd_set_tau(ctx, tau, tau0) bind(C, name='mhd_set_tau')
          import :: c_int, c_ptr, c_double

@@ -60,6 +60,8 @@ SIGNATURES = {
     "mhd_get_bxyz": (C.c_int, [C.c_void_p]),
     "mhd_flux_stress": (C.c_int, [C.c_void_p, C.c_int]),
     "mhd_eijk_lfm": (C.c_int, [C.c_void_p, C.c_int]),
+    "mhd_eijk_yee": (C.c_int, [C.c_void_p, C.c_int]),
+    "mhd_flux_stress_old": (C.c_int, [C.c_void_p]),
     "mhd_set_tau": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     "mhd_get_tau": (C.c_int, [C.c_void_p, _D, _D]),
     "mhd_predict_ab": (C.c_int, [C.c_void_p]),
@@ -259,10 +261,14 @@ class Solver:
                 out.append(a)
         return out
 
-    def eijk(self, which=STATE_CURRENT, compute=True):
-        """get_eijk_LFM of the current or the extrapolated state"""
+    def eijk(self, which=STATE_CURRENT, compute=True, yee=False):
+        """get_eijk_LFM (or, yee=True, the reference's alternative
+        get_eijk_Yee) of the current or the extrapolated state"""
         if compute:
-            _ck(lib().mhd_eijk_lfm(self.h, int(which)), "mhd_eijk_lfm")
+            if yee:
+                _ck(lib().mhd_eijk_yee(self.h, int(which)), "mhd_eijk_yee")
+            else:
+                _ck(lib().mhd_eijk_lfm(self.h, int(which)), "mhd_eijk_lfm")
         g = self.grid
         shp = {FETCH_EI: (g.nx0, g.ny0 + 1, g.nz0 + 1), FETCH_EJ: (g.nx0 + 1, g.ny0, g.nz0 + 1),
                FETCH_EK: (g.nx0 + 1, g.ny0 + 1, g.nz0)}

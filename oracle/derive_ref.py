@@ -22,8 +22,9 @@ Patches applied to the temporary copy (none touches the numerics):
     the reference's own shock-tube initial state is used unchanged;
   * raw dumps (stream, little-endian) of the owned state after
     initialization and after every frame, and of the per-routine outputs
-    (get_eijk_LFM, get_flux_stress, get_dt) evaluated on the initial state,
-    when a file ``dump.on`` exists in the run directory;
+    (get_eijk_LFM, get_flux_stress, get_dt, and the alternative
+    get_eijk_Yee whose own calls the reference comments out) evaluated on
+    the initial state, when a file ``dump.on`` exists in the run directory;
   * the step loop (mhd3d3.f90:3283-3299) is timed with MPI_Wtime and the
     wall time printed by rank 0 as ``STEPLOOP_SECONDS <t>``;
   * the ``periodic`` variant additionally sets CART_BCs=.TRUE.
@@ -132,6 +133,7 @@ subroutine dump_init_ops
    character(len=96) :: fname
    integer :: u
    real*8 :: tsave, tcfl
+   real*8, allocatable :: sei(:,:,:), sej(:,:,:), sek(:,:,:)
    logical, external :: dump_enabled
    if (.not. dump_enabled()) return
    tsave = tau
@@ -148,6 +150,23 @@ subroutine dump_init_ops
    write(u) flux_rhoy, flux_rvxy, flux_rvyy, flux_rvzy, flux_engy, stress_xy, stress_yy, stress_zy
    write(u) flux_rhoz, flux_rvxz, flux_rvyz, flux_rvzz, flux_engz, stress_xz, stress_yz, stress_zz
    close(u)
+!..the reference's alternative EMF get_eijk_Yee on the same state (its own
+!..calls are commented out); ei, ej, ek are put back afterwards
+   allocate(sei(size(ei,1),size(ei,2),size(ei,3)), sej(size(ej,1),size(ej,2),size(ej,3)), &
+            sek(size(ek,1),size(ek,2),size(ek,3)))
+   sei = ei
+   sej = ej
+   sek = ek
+   call get_eijk_Yee(rho,vx,vy,vz,p,bx,by,bz)
+   write(fname,'(A,I0,A)') 'dump_yee_r', myrank, '.bin'
+   open(newunit=u, file=trim(fname), access='stream', form='unformatted', status='replace')
+   write(u) xID, yID, zID, nx0, ny0, nz0
+   write(u) ei, ej, ek
+   close(u)
+   ei = sei
+   ej = sej
+   ek = sek
+   deallocate(sei, sej, sek)
 end subroutine dump_init_ops
 !<<<<< oracle harness additions <<<<<
 """

@@ -992,6 +992,90 @@ void orc_eijk_state(orc_state *s, int which) {
         orc_eijk(s);
 }
 
+/* --------------------------------------------------------------------- */
+/* get_eijk_Yee (mhd3d3.f90:1654-1752) with get_v_center (2562-2592)       */
+/* --------------------------------------------------------------------- */
+
+/* reconstruct_3d of a whole array into (l1, l2, l3) (mhd3d3.f90:833-860) */
+static void recon_arr(const orc_params *p, const A3 *f, int dir, A3 *fl, A3 *fr) {
+    orc_reconstruct(p, f->d, f->n1, f->n2, f->n3, fl->n1, fl->n2, fl->n3, dir, fl->d, fr->d);
+}
+
+static void eijk_yee(orc_state *s, const A3 *lrho, const A3 *lvx, const A3 *lvy, const A3 *lvz, const A3 *lp,
+                     const A3 *lbx, const A3 *lby, const A3 *lbz) {
+    const int nx = s->nx, ny = s->ny, nz = s->nz, ng = s->p.ng;
+    const int nx0 = s->p.nx0, ny0 = s->p.ny0, nz0 = s->p.nz0;
+    const size_t n = (size_t)nx * ny * nz;
+    /* split fields e[a][pp, np, pn, nn] (1669-1691): first split the
+     * electric field: bsq, lam, get_v_center's moments per axis, then
+     * e.g. ezpp = lbx*vx0p*vy1p - lby*vx1p*vy0p */
+    A3 e[3][4];
+    for (int a = 0; a < 3; ++a)
+        for (int q = 0; q < 4; ++q) e[a][q] = a3_new(nx, ny, nz);
+    const A3 *vv[3] = {lvx, lvy, lvz}, *bb[3] = {lbx, lby, lbz};
+#pragma omp parallel for schedule(static)
+    for (size_t t = 0; t < n; ++t) {
+        double bsq = lbx->d[t] * lbx->d[t] + lby->d[t] * lby->d[t] + lbz->d[t] * lbz->d[t];
+        double lam = lrho->d[t] / (2. * lp->d[t] + bsq);
+        double m0[3][2], m1[3][2];
+        for (int a = 0; a < 3; ++a) vface_pt(lam, vv[a]->d[t], &m0[a][0], &m0[a][1], &m1[a][0], &m1[a][1]);
+        for (int a = 0; a < 3; ++a) {
+            int b = (a + 1) % 3, c = (a + 2) % 3;
+            for (int q = 0; q < 4; ++q) {
+                int s1 = q & 1, s2 = q >> 1; /* pp, np, pn, nn */
+                e[a][q].d[t] = bb[b]->d[t] * m0[b][s1] * m1[c][s2] - bb[c]->d[t] * m1[b][s1] * m0[c][s2];
+            }
+        }
+    }
+    /* per edge direction: (first transverse direction, its extents),
+     * (second, its extents) -- ei: y then z (1696-1714), ej: z then x
+     * (1722-1739), ek: x then y (1747-1764); 5 = y, 6 = z, 4 = x untruncated */
+    const int dir1[3] = {5, 6, 4}, dir2[3] = {6, 4, 5};
+    const int l1[3][3] = {{nx, ny0 + 1, nz}, {nx, ny, nz0 + 1}, {nx0 + 1, ny, nz}};
+    const int l2[3][3] = {{nx, ny0 + 1, nz0 + 1}, {nx0 + 1, ny, nz0 + 1}, {nx0 + 1, ny0 + 1, nz}};
+    A3 *out[3] = {&s->ei, &s->ej, &s->ek};
+    for (int a = 0; a < 3; ++a) {
+        A3 q[4], d1 = a3_new(l1[a][0], l1[a][1], l1[a][2]);
+        for (int k = 0; k < 4; ++k) q[k] = a3_new(l1[a][0], l1[a][1], l1[a][2]);
+        /* qepp, qepn: left states; qenn, qenp: right states */
+        recon_arr(&s->p, &e[a][0], dir1[a], &q[0], &d1);
+        recon_arr(&s->p, &e[a][2], dir1[a], &q[2], &d1);
+        recon_arr(&s->p, &e[a][3], dir1[a], &d1, &q[3]);
+        recon_arr(&s->p, &e[a][1], dir1[a], &d1, &q[1]);
+        A3 epp = a3_new(l2[a][0], l2[a][1], l2[a][2]), enp = a3_new(l2[a][0], l2[a][1], l2[a][2]);
+        A3 enn = a3_new(l2[a][0], l2[a][1], l2[a][2]), epn = a3_new(l2[a][0], l2[a][1], l2[a][2]);
+        A3 d2 = a3_new(l2[a][0], l2[a][1], l2[a][2]);
+        recon_arr(&s->p, &q[0], dir2[a], &epp, &d2);  /* epp: left of qepp */
+        recon_arr(&s->p, &q[1], dir2[a], &enp, &d2);  /* enp: left of qenp */
+        recon_arr(&s->p, &q[3], dir2[a], &d2, &enn);  /* enn: right of qenn */
+        recon_arr(&s->p, &q[2], dir2[a], &d2, &epn);  /* epn: right of qepn */
+        /* ei(i,:,:)=epp(ng+i,:,:)+enp(ng+i,:,:)+epn(ng+i,:,:)+enn(ng+i,:,:) (1717) and kin */
+        A3 *o = out[a];
+        for (int k = 1; k <= o->n3; ++k)
+            for (int j = 1; j <= o->n2; ++j)
+                for (int i = 1; i <= o->n1; ++i) {
+                    int I = i + (a == 0 ? ng : 0), J = j + (a == 1 ? ng : 0), K = k + (a == 2 ? ng : 0);
+                    AT(*o, i, j, k) = AT(epp, I, J, K) + AT(enp, I, J, K) + AT(epn, I, J, K) + AT(enn, I, J, K);
+                }
+        for (int k = 0; k < 4; ++k) a3_free(&q[k]);
+        a3_free(&d1);
+        a3_free(&epp);
+        a3_free(&enp);
+        a3_free(&enn);
+        a3_free(&epn);
+        a3_free(&d2);
+    }
+    for (int a = 0; a < 3; ++a)
+        for (int q = 0; q < 4; ++q) a3_free(&e[a][q]);
+}
+
+void orc_eijk_yee_state(orc_state *s, int which) {
+    if (which == 1)
+        eijk_yee(s, &s->rhoh, &s->vxh, &s->vyh, &s->vzh, &s->ph, &s->bxh, &s->byh, &s->bzh);
+    else
+        eijk_yee(s, &s->rho, &s->vx, &s->vy, &s->vz, &s->pr, &s->bx, &s->by, &s->bz);
+}
+
 /* get_conserve and the stage update from the held fluxes and EMFs: stage 0
  * the Adams-Bashforth corrector, 1..nst a TVD RK stage (stage 1 after the
  * advance_previous that step_rk2 takes just before it, mhd3d3.f90:3664) */

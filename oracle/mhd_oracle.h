@@ -60,6 +60,8 @@ void orc_get_tau(const orc_state *s, double *tau, double *tau0);
 int orc_predict_ab(orc_state *s);
 void orc_flux_stress_state(orc_state *s, int which);
 void orc_eijk_state(orc_state *s, int which);
+/* get_eijk_Yee (mhd3d3.f90:1654-1752) of the current (0) or extrapolated (1) state */
+void orc_eijk_yee_state(orc_state *s, int which);
 int orc_advance(orc_state *s, int stage);
 void orc_set_ops(orc_state *s, const double *ei, const double *ej, const double *ek, const double *flux_all);
 /* nsteps calls of `step` (mhd3d3.f90:3585); *tau in/out as the deck tau / tau0. */

@@ -59,6 +59,7 @@ def lib():
         L.orc_predict_ab.restype = C.c_int
         L.orc_flux_stress_state.argtypes = [C.c_void_p, C.c_int]
         L.orc_eijk_state.argtypes = [C.c_void_p, C.c_int]
+        L.orc_eijk_yee_state.argtypes = [C.c_void_p, C.c_int]
         L.orc_advance.argtypes = [C.c_void_p, C.c_int]
         L.orc_advance.restype = C.c_int
         L.orc_set_ops.argtypes = [C.c_void_p, _D, _D, _D, _D]
@@ -163,12 +164,16 @@ class Oracle:
         arrs = [np.asfortranarray(ops[k], dtype=np.float64) for k in ("ei", "ej", "ek")]
         lib().orc_set_ops(self.h, *[_p(a) for a in arrs], _p(flat))
 
-    def ops(self, which=0, compute=True):
-        """run get_eijk_LFM + get_flux_stress on the current state (which=0)
-        or the Adams-Bashforth extrapolation (which=1) and return the
-        products; compute=False returns the held products as they are."""
+    def ops(self, which=0, compute=True, yee=False):
+        """run get_eijk_LFM (yee=True: get_eijk_Yee) + get_flux_stress on the
+        current state (which=0) or the Adams-Bashforth extrapolation
+        (which=1) and return the products; compute=False returns the held
+        products as they are."""
         if compute:
-            lib().orc_eijk_state(self.h, int(which))
+            if yee:
+                lib().orc_eijk_yee_state(self.h, int(which))
+            else:
+                lib().orc_eijk_state(self.h, int(which))
             lib().orc_flux_stress_state(self.h, int(which))
         g = self.g
         n0 = (g.nx0, g.ny0, g.nz0)

@@ -11,6 +11,8 @@ per-rank raw dumps into one ``<case>.npz`` holding
   ops_tcfl        get_dt on the initial state
   ops_ei/ej/ek    get_eijk_LFM on the initial state
   ops_flux<n>     the 24 get_flux_stress outputs on the initial state
+  yee_ei/ej/ek    get_eijk_Yee (the reference's alternative EMF) on the
+                  initial state
   f<n>_<v>        owned state after frame n (time, tau in f<n>_time/_tau)
 
 Two ranks per direction are used throughout: with one rank the reference
@@ -158,6 +160,25 @@ def assemble_ops(run, N):
     return ops
 
 
+def assemble_yee(run, N):
+    """get_eijk_Yee on the initial state (the reference's alternative EMF)"""
+    nx, ny, nz = N
+    out = {"ei": np.full((nx, ny + 1, nz + 1), np.nan, order="F"),
+           "ej": np.full((nx + 1, ny, nz + 1), np.nan, order="F"),
+           "ek": np.full((nx + 1, ny + 1, nz), np.nan, order="F")}
+    for r in range(8):
+        hdr, buf = _read_rank(os.path.join(run, "dump_yee_r%d.bin" % r))
+        xid, yid, zid, nx0, ny0, nz0 = (int(q) for q in hdr)
+        st = (xid * nx0, yid * ny0, zid * nz0)
+        off = 0
+        for name, e in (("ei", (0, 1, 1)), ("ej", (1, 0, 1)), ("ek", (1, 1, 0))):
+            loc, off = _take(buf, off, (nx0 + e[0], ny0 + e[1], nz0 + e[2]))
+            _place(out[name], loc, st, "yee_" + name)
+    for k, a in out.items():
+        assert not np.isnan(a).any(), k
+    return out
+
+
 def run_case(name):
     dom, kind, seed, reconop, stepop, tau, nframes, keep, nord = CASES[name]
     ng = nord // 2
@@ -186,6 +207,8 @@ def run_case(name):
         ops = assemble_ops(run, GRID)
         for k, a in ops.items():
             res["ops_" + k] = a
+        for k, a in assemble_yee(run, GRID).items():
+            res["yee_" + k] = a
         for n in keep:
             fr, t, tu = assemble_state(run, "f%d" % n, GRID)
             for v in STATE:

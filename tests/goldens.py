@@ -43,6 +43,11 @@ class Case:
     def frame(self, n, v):
         return self.z["f%d_%s" % (n, v)]
 
+    def yee(self):
+        """get_eijk_Yee on the initial state (the reference's alternative EMF)"""
+        z = self.z
+        return dict(ei=z["yee_ei"], ej=z["yee_ej"], ek=z["yee_ek"])
+
     def ops(self):
         z = self.z
         return dict(tcfl=float(z["ops_tcfl"]), ei=z["ops_ei"], ej=z["ops_ej"], ek=z["ops_ek"],

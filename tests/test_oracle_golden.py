@@ -48,6 +48,32 @@ def test_initial_ops_bit_exact(name):
 
 
 @pytest.mark.parametrize("name", ALL)
+def test_initial_yee_bit_exact(name):
+    """get_eijk_Yee (mhd3d3.f90:1654-1752), the reference's alternative EMF
+    whose own calls are commented out, on the initial state: the fixtures
+    hold it from the reference build (its harness calls it once there), all
+    four reconstructions (each Yee edge value takes eight reconstruct_3d
+    calls in directions 4-6 on top of the Gaussian split of get_v_center)."""
+    c = Case(name)
+    o, _ = _oracle(c)
+    ref = c.yee()
+    ops = o.ops(yee=True)
+    for k in ("ei", "ej", "ek"):
+        assert np.array_equal(ops[k], ref[k]), k
+
+
+def test_yee_differs_from_lfm():
+    """the two EMFs are different routines: on a magnetised flow they must
+    not coincide (a Yee entry that fell back to get_eijk_LFM would pass the
+    comparison above only where both vanish)"""
+    c = Case("ot_pdm_rk2_per")
+    lfm, yee = c.ops(), c.yee()
+    for k in ("ei", "ej", "ek"):
+        d = np.abs(lfm[k] - yee[k]).max()
+        assert d > 1e-6 * np.abs(lfm[k]).max(), k
+
+
+@pytest.mark.parametrize("name", ALL)
 def test_trajectory_bit_exact(name):
     """the owned state after each stored frame (nts=1: one step per frame)."""
     c = Case(name)
