@@ -519,28 +519,59 @@ __global__ void __launch_bounds__(kBlock) k_emf_march(Dims d, Coef<R> c, EmfIn<T
     }
     const unsigned ou = K == 1 ? osy : osx, ow = K == 3 ? osy : osz;
     R f_uw = R(bvf.at(obc - ou - ow, 0u)), f_w = R(bvf.at(obc - ou - ow, ou));
+    // a step's loads: the taps of the reconstruction across the march, the
+    // march one's new tap and the two new cells; fp32 loads them one step
+    // ahead, so their latency hides behind the current step's arithmetic
+    struct StepTaps {
+        R o[S::n];
+        R nm, f0, fu;
+    };
+    auto load_taps = [&](unsigned o1s, unsigned o2s, unsigned ocs, StepTaps& t) {
+        if constexpr (K == 1) {
+            const unsigned o0 = o2s + (unsigned)S::lo * ost2;
+#pragma unroll
+            for (int m = 0; m < S::n; ++m) t.o[m] = R(bb2.at(o0, (unsigned)m * ost2));
+        } else {
+            const TI* q = reinterpret_cast<const TI*>(reinterpret_cast<const char*>(K == 2 ? in.b1 : in.b2) +
+                                                      (K == 2 ? o1s : o2s));
+#pragma unroll
+            for (int m = 0; m < S::n; ++m) t.o[m] = R(q[S::lo + m]);
+        }
+        t.nm = R(bma.at((MA1 ? o1s : o2s) + (unsigned)(S::lo + S::n - 1) * ostma, 0u));
+        t.f0 = R(bvf.at(ocs, 0u));
+        t.fu = R(bvf.at(ocs - ou, 0u));
+    };
+    constexpr bool PREB = sizeof(R) == 4;
+    StepTaps cur;
+    load_taps(ob1, ob2, obc, cur);
 #pragma unroll
     for (int s = 0; s < T; ++s) {
         {
+            StepTaps nxt;
+            if constexpr (PREB) {
+                if (s + 1 < T) load_taps(ob1 + omv1, ob2 + omv2, obc + omarch, nxt);
+            }
             if (s > 0) {
 #pragma unroll
                 for (int m = 0; m < S::n - 1; ++m) ta[m] = ta[m + 1];
-                ta[S::n - 1] = R(bma.at((MA1 ? ob1 : ob2) + (unsigned)(S::lo + S::n - 1) * ostma, 0u));
+                ta[S::n - 1] = cur.nm;
             }
             const R sig1 = sgn1(v1av[s]), sig2 = sgn1(v2av[s]);
             R b1l, b1r, b2l, b2r;
-            if constexpr (MA1) S::run(ta, c, b1l, b1r);
-            else if constexpr (K == 2) recon_off<R, M, true>(in.b1, ob1, ost1, c, b1l, b1r);
-            if constexpr (!MA1) S::run(ta, c, b2l, b2r);
-            else if constexpr (K == 3) recon_off<R, M, true>(in.b2, ob2, ost2, c, b2l, b2r);
-            else recon_buf<R, M>(bb2, ob2, ost2, c, b2l, b2r);
+            if constexpr (MA1) {
+                S::run(ta, c, b1l, b1r);
+                S::run(cur.o, c, b2l, b2r);
+            } else {
+                S::run(cur.o, c, b1l, b1r);
+                S::run(ta, c, b2l, b2r);
+            }
             const R b1u = pick(sig2, b1l, b1r);
             const R b2u = pick(sig1, b2l, b2r);
             R dum = -v1av[s] * b2u + v2av[s] * b1u;
             // the four cells around the edge (offsets u and w along its
             // transverse axes, w the march axis): (u-1,w-1) and (u,w-1) are
             // the previous edge's (u-1,w) and (u,w)
-            const R f_0 = R(bvf.at(obc, 0u)), f_u = R(bvf.at(obc - ou, 0u));
+            const R f_0 = cur.f0, f_u = cur.fu;
             R eta;
             if (K == 1)
                 eta = R(.25) * (f_uw + f_w + f_0 + f_u);
@@ -557,6 +588,10 @@ __global__ void __launch_bounds__(kBlock) k_emf_march(Dims d, Coef<R> c, EmfIn<T
             TF ev = TF(dum * (efa ? R(efa[eo]) : ef));
             step_fence(ev);
             if (s >= skip) e[eo] = ev;
+            if (s + 1 < T) {
+                if constexpr (!PREB) load_taps(ob1 + omv1, ob2 + omv2, obc + omarch, nxt);
+                cur = nxt;
+            }
             ob1 += omv1;
             ob2 += omv2;
             obc += omarch;
