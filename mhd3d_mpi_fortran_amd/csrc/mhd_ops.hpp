@@ -44,7 +44,8 @@ void flux_launch(mhd_ctx* c, Src<TI> s) {
 // than c->emf_fill takes runs of 2 instead, trading window reuse for latency
 // hiding.  PDM's ei march (both corner axes strided, its loads the slowest)
 // takes runs of 16: at 256^3 0.20 vs 0.22 ms (fp32) and 0.48 vs 0.51 ms
-// (binary64); for ej and ek runs of 16 measured slower (0.19 vs 0.17 ms fp32).
+// (binary64), and runs of 32 are slower again (0.23 ms fp32, 0.83 ms
+// binary64); for ej and ek runs of 16 measured slower (0.19 vs 0.17 ms fp32).
 template <int M, int K>
 struct EmfSteps {
     static constexpr int T = (M == TVD || (M == PDM && K == 1)) ? 16 : 8;
