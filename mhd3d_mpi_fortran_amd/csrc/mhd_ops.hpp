@@ -45,7 +45,10 @@ void flux_launch(mhd_ctx* c, Src<TI> s) {
 // hiding.  PDM's ei march (both corner axes strided, its loads the slowest)
 // takes runs of 16: at 256^3 0.20 vs 0.22 ms (fp32) and 0.48 vs 0.51 ms
 // (binary64), and runs of 32 are slower again (0.23 ms fp32, 0.83 ms
-// binary64); for ej and ek runs of 16 measured slower (0.19 vs 0.17 ms fp32).
+// binary64); for ej and ek runs of 16 measured slower (0.19 vs 0.17 ms
+// fp32), and so did runs of 4 (0.17 vs 0.16 ms) and register caps that
+// buy a fifth or sixth wavefront per SIMD with spills (EMF class 0.61 and
+// 0.72 vs 0.53 ms).
 template <int M, int K>
 struct EmfSteps {
     static constexpr int T = (M == TVD || (M == PDM && K == 1)) ? 16 : 8;
