@@ -63,8 +63,10 @@ MLP_FLOPS = 2 * (40 * 32 + 32 * 32 + 32 * 16)
 # WRITE_SIZE passes committed under profiles/ (same workload)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "blast256_sheared_traffic.json")
 TRAFFIC_KERNELS = {("flux", "mixed", True): ["sur::k_flux_sur<%d, 1>" % d for d in (1, 2, 3)],
-                   ("flux", "mixed", False): ["mhd::k_flux<float, 1, %d, float, float, 1>" % d for d in (1, 2, 3)],
-                   ("flux", "fp64", False): ["mhd::k_flux<double, 1, %d, double, double, 1>" % d for d in (1, 2, 3)],
+                   # the fp32 y and z sweeps march 8 face tiles per workgroup at 256^3
+                   ("flux", "mixed", False): ["mhd::k_flux<float, 1, 1, float, float, 1, 1>"] +
+                                             ["mhd::k_flux<float, 1, %d, float, float, 1, 8>" % d for d in (2, 3)],
+                   ("flux", "fp64", False): ["mhd::k_flux<double, 1, %d, double, double, 1, 1>" % d for d in (1, 2, 3)],
                    # PDM's ei march takes runs of 16 edges, ej and ek runs of 8
                    ("emf", "mixed", True): ["mhd::k_emf_march<float, 1, %d, %d, float, float>" % dt
                                             for dt in ((1, 16), (2, 8), (3, 8))],

@@ -296,47 +296,67 @@ __device__ __forceinline__ void gen_normal(const Dims& d, const GMet& g, int i, 
     n[2] = R(a[2 * nb + f]);
 }
 
-// fp32 strided sweeps (D = 2, 3): the face tile's 64 x-columns by
-// kTileN + n - 1 rows along the sweep (its 4 face planes and their n taps)
-// are loaded into LDS once, each row coalesced, and every face reconstructs
-// from there; the wavefronts would otherwise fetch each row n times over.
-// Every thread of the workgroup calls it (a barrier); columns and rows past
-// the block's edges are clamped onto it (their faces are not stored).
-template <typename R, int M, int D, typename TI>
-__device__ __forceinline__ void tile_recon(const Dims& d, const Prim<TI>& q, const Coef<R>& c, unsigned tile, int ii,
-                                           int jj, int kk, R* l, R* r) {
+// fp32 strided sweeps (D = 2, 3): a workgroup takes kFluxRun consecutive face
+// tiles along the sweep of one x-column run (a tile: 64 x-faces by kTileN
+// face planes, one wavefront per plane).  Tile tn needs sweep rows
+// 4 tn .. 4 tn + kTileN + n - 2 (row rho holds cell ng + 1 + lo + rho), so
+// consecutive tiles share n - 1 rows: the rows sit in an LDS ring of 16 (row
+// rho in slot rho mod 16), the first tile loads all of its rows and every
+// later one only its kTileN new ones, each row one coalesced load.  Every
+// thread of the workgroup calls it, tile after tile (two barriers); columns
+// and rows past the block's edges are clamped onto it.  At 256^3 (fp32 PDM,
+// sheared) a march of 8 tiles takes the step from 3.10 ms (one tile per
+// workgroup, 11 rows loaded per 4 planes) to 2.97 ms; 4 and 16 tiles, and a
+// ring of 12 (six workgroups per CU instead of five), measured slower.  A
+// march needs kFluxRun times fewer workgroups, so blocks whose sweep would
+// then leave fewer than kFluxRunBlocks workgroups take one tile each
+// (RUN = 1, the rows in LDS as they come) instead: the shell of 96 x 64 x 128
+// and the ensemble's boxes lost 5-25 % with runs of 8.
+constexpr int kFluxRun = 8, kFluxRing = 16, kFluxRunBlocks = 4096;
+template <int D>
+__host__ __device__ __forceinline__ unsigned face_runs(int ii, int jj, int kk, int run) {
+    const unsigned nn = D == 2 ? jj : kk, no = D == 2 ? kk : jj;
+    const unsigned ntn = (nn + kTileN - 1) / kTileN;
+    return (unsigned)((ii + kTileX - 1) / kTileX) * ((ntn + run - 1) / run) * no;
+}
+template <typename R, int M, int D, typename TI, int RUN>
+__device__ __forceinline__ void ring_recon(const Dims& d, const Prim<TI>& q, const Coef<R>& c, unsigned tx,
+                                           unsigned tn, unsigned o, int ii, unsigned no, bool first, R* l, R* r) {
     using S = Stencil<R, M>;
     constexpr int NR = kTileN + S::n - 1;
-    __shared__ TI win[8][NR][kTileX];
+    constexpr int RING = RUN == 1 ? NR : kFluxRing;
+    static_assert(NR <= kFluxRing, "the ring holds one tile's rows");
+    __shared__ TI win[8][RING][kTileX];
+    // LDS slot of the tile's row rho
+    auto slot = [&](int rho) { return RUN == 1 ? rho : ((int)(tn * kTileN) + rho) & (kFluxRing - 1); };
     const TI* f[8] = {q.rho, q.vx, q.vy, q.vz, q.p, q.bx, q.by, q.bz};
-    const unsigned nn = D == 2 ? jj : kk, no = D == 2 ? kk : jj;
-    const unsigned ntx = (ii + kTileX - 1) / kTileX, ntn = (nn + kTileN - 1) / kTileN;
-    const unsigned tx = tile % ntx, rr = tile / ntx, tn = rr % ntn, o = rr / ntn;
     const int lane = threadIdx.x & (kTileX - 1), w = threadIdx.x / kTileX;
     const int cx = d.ng + 1 + min((int)(tx * kTileX) + lane, ii - 1);
     const int co = d.ng + 1 + min((int)o, (int)no - 1);
     const int cmax = D == 2 ? d.ny : d.nz;
+    const int r0 = (int)(tn * kTileN);
+    __syncthreads();  // the previous tile's reads are done
 #pragma unroll
-    for (int row = w; row < NR; row += kTileN) {
-        // row w + m holds tap m of face plane w: cell ng + 4 tn + 1 + lo + row
-        const int cs = max(1, min(d.ng + (int)(tn * kTileN) + 1 + S::lo + row, cmax));
+    for (int rho = first ? w : NR - kTileN + w; rho < NR; rho += kTileN) {
+        const int cs = max(1, min(d.ng + 1 + S::lo + r0 + rho, cmax));
         const unsigned at = (unsigned)(D == 2 ? ix(d.nx, d.ny, cx, cs, co) : ix(d.nx, d.ny, cx, co, cs));
 #pragma unroll
-        for (int v = 0; v < 8; ++v) win[v][row][lane] = f[v][at];
+        for (int v = 0; v < 8; ++v) win[v][slot(rho)][lane] = f[v][at];
     }
     __syncthreads();
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
         R t[S::n];
 #pragma unroll
-        for (int m = 0; m < S::n; ++m) t[m] = R(win[v][w + m][lane]);
+        for (int m = 0; m < S::n; ++m) t[m] = R(win[v][slot(w + m)][lane]);
         S::run(t, c, l[v], r[v]);
     }
 }
 
 // MET = 0: the reference's axis-aligned flux; 1: along the affine metric's
 // normals; 2: along the general metric's per-face normals
-template <typename R, int M, int D, typename TI, typename TF, int MET>
+// RUN: face tiles per workgroup of the fp32 strided sweeps
+template <typename R, int M, int D, typename TI, typename TF, int MET, int RUN = 1>
 __global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<TI> q, Normals<R> nm, GMet gm,
                                                  TF* __restrict__ F, long nface) {
     const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2), kk = d.nz0 + (D == 3);
@@ -360,12 +380,35 @@ __global__ void __launch_bounds__(kBlock) k_flux(Dims d, Coef<R> c, Prim<TI> q, 
         recon_line<R, M>(q.by, base, st, c, l[6], r[6]);
         recon_line<R, M>(q.bz, base, st, c, l[7], r[7]);
     } else if constexpr (sizeof(TI) == 4) {
-        // fp32 stencil inputs: the tile's rows through LDS
-        const unsigned tile = xcd_block();
-        const bool live = tile_face<D>(tile, ii, jj, kk, i, j, k);
-        tile_recon<R, M, D, TI>(d, q, c, tile, ii, jj, kk, l, r);
-        if (!live) return;
-        t = ix(ii, jj, i, j, k);
+        // fp32 stencil inputs: RUN tiles along the sweep, their rows
+        // through the LDS ring
+        const unsigned nn = D == 2 ? jj : kk, no = D == 2 ? kk : jj;
+        const unsigned ntx = (ii + kTileX - 1) / kTileX, ntn = (nn + kTileN - 1) / kTileN;
+        const unsigned ntc = (ntn + RUN - 1) / RUN;
+        const unsigned lb = xcd_block<kXcdRun / RUN>();
+        const unsigned tx = lb % ntx, tnc = lb / ntx % ntc, o = lb / ntx / ntc;
+        const unsigned x = tx * kTileX + (threadIdx.x & (kTileX - 1));
+#pragma unroll 1
+        for (int g = 0; g < RUN; ++g) {
+            const unsigned tn = tnc * RUN + g, sw = tn * kTileN + threadIdx.x / kTileX;
+            ring_recon<R, M, D, TI, RUN>(d, q, c, tx, tn, o, ii, no, g == 0, l, r);
+            if (x < (unsigned)ii && sw < nn) {
+                i = (int)x + 1;
+                j = (int)(D == 2 ? sw : o) + 1;
+                k = (int)(D == 2 ? o : sw) + 1;
+                t = ix(ii, jj, i, j, k);
+                if (MET == 2) {
+                    R n[3];
+                    gen_normal<R, D>(d, gm, i, j, k, n);
+                    face_flux_met<R, TF>(c, l, r, n, F, t, nface);
+                } else if (MET == 1) {
+                    face_flux_met<R, TF>(c, l, r, nm.n[D - 1], F, t, nface);
+                } else {
+                    face_flux<R, D, TF>(c, l, r, F, t, nface);
+                }
+            }
+        }
+        return;
     } else {
         // binary64 (VALU-bound): taps from the tile's rows in L1
         if (!tile_face<D>(xcd_block(), ii, jj, kk, i, j, k)) return;

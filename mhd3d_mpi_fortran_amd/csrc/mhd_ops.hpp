@@ -16,10 +16,23 @@ void flux_launch_mm(mhd_ctx* c, Src<TI> s) {
     launch(c, MHD_K_FLUX, blocks(c->nf[0]), dim3(kBlock), k_flux<R, M, 1, TI, TF, MET>, c->d, co, q, nm, c->gm,
            (TF*)c->flux[0], c->nf[0]);
     const Dims& d = c->d;
-    launch(c, MHD_K_FLUX, dim3(face_tiles<2>(d.nx0, d.ny0 + 1, d.nz0)), dim3(kBlock), k_flux<R, M, 2, TI, TF, MET>,
-           c->d, co, q, nm, c->gm, (TF*)c->flux[1], c->nf[1]);
-    launch(c, MHD_K_FLUX, dim3(face_tiles<3>(d.nx0, d.ny0, d.nz0 + 1)), dim3(kBlock), k_flux<R, M, 3, TI, TF, MET>,
-           c->d, co, q, nm, c->gm, (TF*)c->flux[2], c->nf[2]);
+    // fp32 inputs march kFluxRun tiles per workgroup where that still leaves
+    // kFluxRunBlocks workgroups, else one tile each (k_flux)
+    const unsigned y8 = face_runs<2>(d.nx0, d.ny0 + 1, d.nz0, kFluxRun);
+    const unsigned z8 = face_runs<3>(d.nx0, d.ny0, d.nz0 + 1, kFluxRun);
+    const bool fp32 = sizeof(TI) == 4;
+    if (fp32 && y8 >= (unsigned)kFluxRunBlocks)
+        launch(c, MHD_K_FLUX, dim3(y8), dim3(kBlock), k_flux<R, M, 2, TI, TF, MET, kFluxRun>, c->d, co, q, nm,
+               c->gm, (TF*)c->flux[1], c->nf[1]);
+    else
+        launch(c, MHD_K_FLUX, dim3(face_tiles<2>(d.nx0, d.ny0 + 1, d.nz0)), dim3(kBlock),
+               k_flux<R, M, 2, TI, TF, MET>, c->d, co, q, nm, c->gm, (TF*)c->flux[1], c->nf[1]);
+    if (fp32 && z8 >= (unsigned)kFluxRunBlocks)
+        launch(c, MHD_K_FLUX, dim3(z8), dim3(kBlock), k_flux<R, M, 3, TI, TF, MET, kFluxRun>, c->d, co, q, nm,
+               c->gm, (TF*)c->flux[2], c->nf[2]);
+    else
+        launch(c, MHD_K_FLUX, dim3(face_tiles<3>(d.nx0, d.ny0, d.nz0 + 1)), dim3(kBlock),
+               k_flux<R, M, 3, TI, TF, MET>, c->d, co, q, nm, c->gm, (TF*)c->flux[2], c->nf[2]);
 }
 
 template <typename R, typename TI, typename TF, int M>
