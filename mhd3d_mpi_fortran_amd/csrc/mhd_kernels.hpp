@@ -335,7 +335,7 @@ __device__ __forceinline__ void ring_recon(const Dims& d, const Prim<TI>& q, con
     const int co = d.ng + 1 + min((int)o, (int)no - 1);
     const int cmax = D == 2 ? d.ny : d.nz;
     const int r0 = (int)(tn * kTileN);
-    __syncthreads();  // the previous tile's reads are done
+    if (RUN > 1) __syncthreads();  // the previous tile's reads are done
 #pragma unroll
     for (int rho = first ? w : NR - kTileN + w; rho < NR; rho += kTileN) {
         const int cs = max(1, min(d.ng + 1 + S::lo + r0 + rho, cmax));

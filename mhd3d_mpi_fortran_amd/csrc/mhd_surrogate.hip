@@ -133,19 +133,18 @@ struct State {
     int* net = nullptr;                   // net-list segments
     unsigned long long* ctr = nullptr;    // the segments' lengths
     long seg_n = 0;                       // capacity in segments (all directions)
+    long* blk = nullptr;                  // sample collection: per-workgroup counts and first slots
+    long blk_n = 0;
     long step = 0;
     bool have_weights = false;
 };
 
-// ---------------------------------------------------------------------------
-// sample collection: faces of the current (binary64) state along direction D
-// ---------------------------------------------------------------------------
+// the sample of face t along D: the normalised windows x and the PDM face
+// values y (targets); false if the face fails the smoothness test.  With
+// full = false only x and the test are formed.
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_collect(Dims d, Coef<double> c, Prim<double> q, float theta, int stride,
-                                                   int phase, float* __restrict__ out, long cap,
-                                                   long* __restrict__ count, long nface) {
-    const long t = gtid();
-    if (t >= nface || (t % stride) != phase) return;
+__device__ __forceinline__ bool collect_face(const Dims& d, const Coef<double>& c, const Prim<double>& q, float theta,
+                                             long t, bool full, float* x, float* y) {
     const int ii = d.nx0 + (D == 1), jj = d.ny0 + (D == 2);
     int i, j, k;
     unflat(t, ii, jj, i, j, k);
@@ -153,7 +152,7 @@ __global__ void __launch_bounds__(kBlock) k_collect(Dims d, Coef<double> c, Prim
     const ptrdiff_t st = D == 1 ? 1 : (D == 2 ? (ptrdiff_t)d.nx : (ptrdiff_t)d.nx * d.ny);
     const double* f[8] = {q.rho, q.vx, q.vy, q.vz, q.p, q.bx, q.by, q.bz};
     double wd[NI], cc[8], mm[8], sc[8];
-    float x[NI], y[NO], rough = 0.f;
+    float rough = 0.f;
 #pragma unroll
     for (int v = 0; v < 8; ++v) load5<double>(f[v], base, st, wd + 5 * v, cc[v], mm[v]);
     var_scales<double>(cc, sc);
@@ -163,20 +162,87 @@ __global__ void __launch_bounds__(kBlock) k_collect(Dims d, Coef<double> c, Prim
 #pragma unroll
         for (int a = 0; a < 5; ++a) x[5 * v + a] = float((wd[5 * v + a] - cc[v]) * r);
         rough = fmaxf(rough, rough5(x + 5 * v));
-        double fl, fr;
-        recon_line<double, PDM>(f[v], base, st, c, fl, fr);
-        y[v] = float((fl - cc[v]) * r);
-        y[8 + v] = float((fr - cc[v]) * r);
+        if (full) {
+            double fl, fr;
+            recon_line<double, PDM>(f[v], base, st, c, fl, fr);
+            y[v] = float((fl - cc[v]) * r);
+            y[8 + v] = float((fr - cc[v]) * r);
+        }
     }
-    if (!(rough <= theta)) return;
-    const long slot = atomicAdd((unsigned long long*)count, 1ull);
-    if (slot >= cap) return;
-    float* o = out + slot * NS;
-#pragma unroll
-    for (int a = 0; a < NI; ++a) o[a] = x[a];
-#pragma unroll
-    for (int a = 0; a < NO; ++a) o[NI + a] = y[a];
+    return rough <= theta;
 }
+
+// ---------------------------------------------------------------------------
+// sample collection: faces of the current (binary64) state along direction D,
+// in face order.  Pass 0 counts each workgroup's faces that pass the
+// smoothness test, k_collect_scan turns the counts into each workgroup's
+// first slot (after the samples already held), pass 1 writes the samples:
+// the training set -- and with it the trained net -- is the same from run to
+// run (a slot per atomic increment would order the samples, and choose the
+// ones kept when the set fills up, by the race).
+// ---------------------------------------------------------------------------
+template <int D, int PASS>
+__global__ void __launch_bounds__(kBlock) k_collect(Dims d, Coef<double> c, Prim<double> q, float theta, int stride,
+                                                   int phase, float* __restrict__ out, long cap,
+                                                   long* __restrict__ cnt, const long* __restrict__ first,
+                                                   long nface) {
+    __shared__ int wsum[kBlock / 64];
+    const long t = gtid();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    bool keep = t < nface && (t % stride) == phase;
+    float x[NI], y[NO];
+    if (keep) keep = collect_face<D>(d, c, q, theta, t, PASS == 1, x, y);
+    const unsigned long long m = __ballot(keep);
+    if (lane == 0) wsum[w] = __popcll(m);
+    __syncthreads();
+    if constexpr (PASS == 0) {
+        if (threadIdx.x == 0) {
+            int n = 0;
+            for (int q2 = 0; q2 < (int)(blockDim.x >> 6); ++q2) n += wsum[q2];
+            cnt[blockIdx.x] = n;
+        }
+        return;
+    } else {
+        if (!keep) return;
+        long slot = first[blockIdx.x] + __popcll(m & ((1ull << lane) - 1ull));
+        for (int q2 = 0; q2 < w; ++q2) slot += wsum[q2];
+        if (slot >= cap) return;
+        float* o = out + slot * NS;
+#pragma unroll
+        for (int a = 0; a < NI; ++a) o[a] = x[a];
+#pragma unroll
+        for (int a = 0; a < NO; ++a) o[NI + a] = y[a];
+    }
+}
+
+// exclusive prefix of the n workgroup counts after *count (the samples held
+// so far) into first[]; *count grows by their total.  One workgroup of 1024.
+__global__ void __launch_bounds__(1024) k_collect_scan(const long* __restrict__ cnt, long n, long* __restrict__ first,
+                                                      long* __restrict__ count) {
+    __shared__ long part[1024];
+    const long per = (n + blockDim.x - 1) / blockDim.x, b0 = threadIdx.x * per;
+    const long b1 = b0 + per < n ? b0 + per : n;
+    long s = 0;
+    for (long b = b0; b < b1; ++b) s += cnt[b];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {  // 1024 partial sums, one after the other
+        long run = *count;
+        for (int q = 0; q < (int)blockDim.x; ++q) {
+            const long v = part[q];
+            part[q] = run;
+            run += v;
+        }
+        *count = run;
+    }
+    __syncthreads();
+    long run = part[threadIdx.x];
+    for (long b = b0; b < b1; ++b) {
+        first[b] = run;
+        run += cnt[b];
+    }
+}
+
 
 // ---------------------------------------------------------------------------
 // training
@@ -254,9 +320,13 @@ __global__ void __launch_bounds__(kBlock) k_train_fb(const float* __restrict__ P
 }
 
 // dParam = sum over the batch of delta_out * activation_in; each workgroup
-// covers 64 parameters x 4 batch chunks, grid.y splits the batch further
+// covers 64 parameters x 4 batch chunks, grid.y splits the batch further.
+// Each chunk's partial sum goes to a slot of its own and k_grad_sum adds the
+// kGradChunks slots in chunk order, so the gradient -- and the training --
+// does not depend on the order in which the chunks finish.
+constexpr int kGradChunks = 64;  // = grid.y (16) x 4 wavefronts
 __global__ void __launch_bounds__(kBlock) k_train_grad(const float* __restrict__ scratch, int batch,
-                                                      float* __restrict__ g) {
+                                                      float* __restrict__ gpart) {
     const int pidx = blockIdx.x * 64 + (threadIdx.x & 63);
     const int nch = gridDim.y * 4;
     const int ch = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -275,7 +345,15 @@ __global__ void __launch_bounds__(kBlock) k_train_grad(const float* __restrict__
         const float* sc = scratch + (long)b * SC;
         acc += sc[din] * (ain >= 0 ? sc[ain] : 1.f);
     }
-    atomicAdd(&g[pidx], acc);
+    gpart[(long)ch * NPARAM + pidx] = acc;
+}
+
+__global__ void k_grad_sum(const float* __restrict__ gpart, float* __restrict__ g) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= NPARAM) return;
+    float s = 0.f;
+    for (int ch = 0; ch < kGradChunks; ++ch) s += gpart[(long)ch * NPARAM + a];
+    g[a] = s;
 }
 
 __global__ void k_adam(float* __restrict__ P, float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
@@ -913,7 +991,7 @@ void surrogate_free(mhd_ctx* c) {
     SurrogateState* s = c->sur;
     if (!s) return;
     void* ps[] = {s->p, s->g, s->m, s->v, s->samples, s->count, s->scratch, s->loss, s->wbf, s->bias, s->stats,
-                  s->fix, s->net, s->ctr};
+                  s->fix, s->net, s->ctr, s->blk};
     for (void* p : ps)
         if (p) hipFree(p);
     delete s;
@@ -1064,17 +1142,33 @@ int mhd_surrogate_collect(mhd_ctx* c, double theta, int stride, long capacity, l
         S->samples = nullptr;
         if (hipMalloc(&S->samples, (size_t)capacity * sur::NS * sizeof(float)) != hipSuccess) return MHD_ERR_HIP;
         S->cap = capacity;
-        hipMemsetAsync(S->count, 0, 64, c->s);
+        if (hipMemsetAsync(S->count, 0, 64, c->s) != hipSuccess) return MHD_ERR_HIP;
     }
     const Coef<double> co = make_coef<double>(c);
     const Prim<double> q{c->rho, c->vx, c->vy, c->vz, c->pr, c->bx, c->by, c->bz};
     const int phase = (int)(S->step++ % stride);
-    launch(c, MHD_K_SURR, blocks(c->nf[0]), dim3(kBlock), sur::k_collect<1>, c->d, co, q, (float)theta, stride,
-           phase, S->samples, S->cap, S->count, c->nf[0]);
-    launch(c, MHD_K_SURR, blocks(c->nf[1]), dim3(kBlock), sur::k_collect<2>, c->d, co, q, (float)theta, stride,
-           phase, S->samples, S->cap, S->count, c->nf[1]);
-    launch(c, MHD_K_SURR, blocks(c->nf[2]), dim3(kBlock), sur::k_collect<3>, c->d, co, q, (float)theta, stride,
-           phase, S->samples, S->cap, S->count, c->nf[2]);
+    // per-workgroup counts and first slots of the largest face family
+    const long nblk = std::max(blocks(c->nf[0]).x, std::max(blocks(c->nf[1]).x, blocks(c->nf[2]).x));
+    if (S->blk_n < nblk) {
+        if (S->blk) hipFree(S->blk);
+        S->blk = nullptr;
+        S->blk_n = 0;
+        if (hipMalloc(&S->blk, 2 * (size_t)nblk * sizeof(long)) != hipSuccess) return MHD_ERR_HIP;
+        S->blk_n = nblk;
+    }
+    long* cnt = S->blk;
+    long* first = S->blk + nblk;
+    auto dir = [&](auto kc, auto kw, long nf) {
+        const dim3 g = blocks(nf);
+        launch(c, MHD_K_SURR, g, dim3(kBlock), kc, c->d, co, q, (float)theta, stride, phase, S->samples, S->cap, cnt,
+               (const long*)first, nf);
+        launch(c, MHD_K_SURR, dim3(1), dim3(1024), sur::k_collect_scan, (const long*)cnt, (long)g.x, first, S->count);
+        launch(c, MHD_K_SURR, g, dim3(kBlock), kw, c->d, co, q, (float)theta, stride, phase, S->samples, S->cap, cnt,
+               (const long*)first, nf);
+    };
+    dir(sur::k_collect<1, 0>, sur::k_collect<1, 1>, c->nf[0]);
+    dir(sur::k_collect<2, 0>, sur::k_collect<2, 1>, c->nf[1]);
+    dir(sur::k_collect<3, 0>, sur::k_collect<3, 1>, c->nf[2]);
     long h = 0;
     if (hipMemcpyAsync(&h, S->count, sizeof(long), hipMemcpyDeviceToHost, c->s) != hipSuccess) return MHD_ERR_HIP;
     int rc = sync_or(c);
@@ -1109,11 +1203,15 @@ int mhd_surrogate_fetch_samples(mhd_ctx* c, float* out, long n) {
     return sync_or(c);
 }
 
+// the minibatch scratch (activations and deltas of batch samples), then the
+// gradient's per-chunk partial sums
 static int ensure_scratch(mhd_ctx* c, SurrogateState* S, long batch) {
     if (S->scratch_n >= batch) return MHD_OK;
     if (S->scratch) hipFree(S->scratch);
     S->scratch = nullptr;
-    if (hipMalloc(&S->scratch, (size_t)batch * sur::SC * sizeof(float)) != hipSuccess) return MHD_ERR_HIP;
+    S->scratch_n = 0;
+    const size_t n = (size_t)batch * sur::SC + (size_t)sur::kGradChunks * sur::NPARAM;
+    if (hipMalloc(&S->scratch, n * sizeof(float)) != hipSuccess) return MHD_ERR_HIP;
     S->scratch_n = batch;
     return MHD_OK;
 }
@@ -1122,8 +1220,10 @@ static void run_minibatch(mhd_ctx* c, SurrogateState* S, const float* P, const f
                           long mult, int batch) {
     launch(c, MHD_K_SURR, blocks(batch), dim3(kBlock), sur::k_train_fb, P, set, n, first, mult, batch,
            1.f / (float)(batch * sur::NO), S->scratch, S->loss);
-    launch(c, MHD_K_SURR, dim3((sur::NPARAM + 63) / 64, 16), dim3(kBlock), sur::k_train_grad,
-           (const float*)S->scratch, batch, S->g);
+    float* gpart = S->scratch + (size_t)S->scratch_n * sur::SC;
+    launch(c, MHD_K_SURR, dim3((sur::NPARAM + 63) / 64, sur::kGradChunks / 4), dim3(kBlock), sur::k_train_grad,
+           (const float*)S->scratch, batch, gpart);
+    launch(c, MHD_K_SURR, dim3((sur::NPARAM + 255) / 256), dim3(256), sur::k_grad_sum, (const float*)gpart, S->g);
 }
 
 static long gcd_l(long a, long b) {

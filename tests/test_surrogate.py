@@ -297,6 +297,26 @@ def test_collected_samples_are_consistent(trained):
 
 
 @pytest.mark.gpu
+def test_training_is_deterministic():
+    """collection writes the samples in face order and the gradient adds its
+    partial sums in a fixed order, so two runs from the same snapshots train
+    the same net bit for bit (also when the set fills up and samples are
+    dropped)"""
+    nets = []
+    for _ in range(2):
+        s, n = sg.build_training_set(n=32, capacity=20000, theta=TRAIN_THETA, stride=3, snapshots=2,
+                                     steps_between=3)
+        smp = sg.fetch_samples(s, min(n, 2048))
+        sg.train(s, epochs=1, batch=1024, lr=5e-3, seed=3)
+        nets.append((n, smp, sg.get_weights(s)))
+        s.close()
+    (n1, s1, w1), (n2, s2, w2) = nets
+    assert n1 == n2 == 20000
+    assert np.array_equal(s1, s2)
+    assert np.array_equal(w1, w2)
+
+
+@pytest.mark.gpu
 def test_trained_net_gains_over_baselines(trained):
     held = trained["held"]
     x, t = held[:, :40], held[:, 40:]
@@ -323,7 +343,8 @@ def test_solver_with_learned_reconstruction(trained):
     its own linear part inside the solver.  At theta = 0.25 the seeded density
     noise of this set-up sends ~99 % of the faces to the PDM fallback, the net
     acts on the rest, and its density L1 sits at or above the linear part's
-    (1.37x trained here on 2^20 samples; 1.00x after 8 epochs on 2^24); both
+    (1.34x trained here on 2^20 samples -- training is deterministic, so
+    the figure is this build's; 1.00x after 8 epochs on 2^24); both
     are two orders of magnitude above fp32 PDM.  The bounds below
     are regression guards on that state, not a claimed gain: the net within
     1.5x of the linear part and below 1e-4, PDM below both."""
